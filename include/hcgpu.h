@@ -1,0 +1,212 @@
+/*
+ * hcgpu.h — C ABI of the MI355X batched path tracker (libhcgpu.so).
+ *
+ * This is the drop-in boundary that HomotopyContinuation.jl's batch path would
+ * `ccall`.  It replaces, for a whole batch of paths, the per-path call
+ *   track(tracker::EndgameTracker, x, t₁; path_number)      src/endgame_tracker.jl:902-911
+ * issued by threaded_solve / serial_solve                      src/solve.jl:603-709
+ * and the per-parameter loop of many_solve                     src/solve.jl:815-881.
+ * Each path's output record mirrors PathResult(endgame_tracker) src/endgame_tracker.jl:847-888.
+ *
+ * All functions return 0 on success and a negative HCG_ERR_* code on failure;
+ * hcg_last_error() then returns a thread-local message.  No C++ exception ever
+ * crosses this boundary.  Per-path numerical failures are not errors: they are
+ * path return codes (EndgameTrackerCode ordinals, src/endgame_tracker.jl:100-117).
+ *
+ * Ownership: the caller owns every host array for the duration of a call; the
+ * library copies inputs to the device and results back.  The library owns device
+ * buffers, hipRTC code objects and streams (freed by the *_destroy calls).
+ * Calls are synchronous.  An hcg_ctx is not thread-safe: use one per host thread.
+ */
+#ifndef HCGPU_H
+#define HCGPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCG_OK 0
+#define HCG_ERR_INVALID_ARGUMENT (-1)
+#define HCG_ERR_COMPILE (-2)
+#define HCG_ERR_HIP (-3)
+#define HCG_ERR_OUT_OF_MEMORY (-4)
+#define HCG_ERR_UNSUPPORTED (-5)
+
+typedef struct hcg_ctx hcg_ctx;
+typedef struct hcg_program hcg_program;
+
+/*
+ * Straight-line program, a field-for-field mirror of ModelKit's InstructionSequence
+ * (src/model_kit/instruction_sequence.jl:1-27).
+ *  instr: ninstr x 6 int32 = (input1, input2, input3, input4, op, output), i.e. the
+ *         isbits layout of `Instruction` (:1-5).  Indices are 1-based tape slots;
+ *         op is the OpType ordinal (src/model_kit/operations.jl:5-49, OP_STOP = 0 ...
+ *         OP_MULMULSUB = 32).  For OP_POW_INT input2 is the integer exponent (:178-180).
+ *  constants: nconst complex numbers (re, im) occupying tape slots 1..nconst.
+ *  params_first/vars_first: 1-based first tape slot of the parameter / variable block.
+ *  t_index: tape slot of the continuation parameter, or 0 for a System tape.
+ *  u_assign: nu pairs (i, k): u[i] = tape[k]   (u_assignments, :22)
+ *  U_assign: nU pairs (l, k): U[l] = tape[k], l a 1-based column-major linear index
+ *            (U_assignments, :232-237).
+ *  tape_space: number of tape slots (tape_space_needed).
+ */
+typedef struct hcg_tape {
+    const int32_t* instr;
+    const double* constants;
+    const int32_t* u_assign;
+    const int32_t* U_assign;
+    int32_t ninstr, nconst;
+    int32_t params_first, nparams;
+    int32_t vars_first, nvars;
+    int32_t t_index, nu;
+    int32_t nU, output_dim;
+    int32_t tape_space, pad_;
+} hcg_tape;
+
+/* One AbstractSystem: the evaluate_and_jacobian! tape (u and U) and the evaluate!
+ * tape (u only; also used for the double-double residual and for taylor!).
+ * fixed_params: nparams complex values for a FixedParameterSystem
+ * (src/systems/fixed_parameter_system.jl:9-50) or NULL when the homotopy drives the
+ * parameters (ParameterHomotopy). */
+typedef struct hcg_system {
+    const hcg_tape* eval_jac;
+    const hcg_tape* eval;
+    const double* fixed_params;
+} hcg_system;
+
+#define HCG_STRAIGHT_LINE 0 /* H = γ t G(x) + (1-t) F(x)  src/homotopies/straight_line_homotopy.jl:10-154 */
+#define HCG_PARAMETER 1     /* H = F(x; t p + (1-t) q)     src/homotopies/parameter_homotopy.jl:10-102 */
+
+typedef struct hcg_homotopy_desc {
+    int32_t kind;
+    int32_t n; /* number of variables = number of equations (square systems) */
+    hcg_system start;  /* G (STRAIGHT_LINE only) */
+    hcg_system target; /* F */
+    double gamma_re, gamma_im;
+} hcg_homotopy_desc;
+
+/* Options: a POD mirror of TrackerOptions + TrackerParameters (src/tracker.jl:45-140),
+ * EndgameOptions (src/endgame_tracker.jl:47-72) and WeightedNormOptions
+ * (src/norm.jl:36-40), plus GPU-only knobs.  hcg_default_options fills the
+ * reference defaults. */
+typedef struct hcg_options {
+    /* TrackerOptions */
+    int32_t max_steps;
+    int32_t extended_precision;
+    double max_step_size;
+    double max_initial_step_size;
+    double min_step_size;
+    double min_rel_step_size;
+    double terminate_cond;
+    /* TrackerParameters */
+    double a, beta_a, beta_omega_p, beta_tau, strict_beta_tau;
+    int32_t min_newton_iters;
+    /* EndgameOptions */
+    int32_t max_endgame_steps;
+    double endgame_start;
+    int32_t max_endgame_extended_steps;
+    int32_t max_winding_number;
+    double min_cond, min_cond_growth, min_coord_growth;
+    int32_t zero_is_at_infinity, at_infinity_check;
+    int32_t only_nonsingular, refine_steps;
+    double singular_min_accuracy;
+    double val_finite_tol, val_at_infinity_tol;
+    double sing_cond, sing_accuracy, scaling_threshold;
+    /* WeightedNormOptions */
+    double scale_min, scale_abs_min, scale_max;
+    /* GPU-only knobs */
+    int32_t learned_step;     /* 1: use the learned step-size proposer (plain rule stays the judge) */
+    int32_t block_size;       /* 0: autotuned */
+    int32_t blocks_per_cu;    /* 0: autotuned */
+    int32_t log_steps;        /* 1: write per-step feature records into hcg_step_log */
+} hcg_options;
+
+/* A batch of paths for one program. */
+typedef struct hcg_batch {
+    int32_t npaths;
+    int32_t ngroups;              /* number of target-parameter groups (PARAMETER) */
+    const double* starts;         /* npaths * n complex, path-major */
+    const double* start_params;   /* nparams complex: p (PARAMETER) */
+    const double* target_params;  /* ngroups * nparams complex: q per group */
+    const int32_t* group_of_path; /* npaths entries in [0, ngroups) or NULL (group 0) */
+    double t1, t0;                /* track from t1 to t0 (EndgameTracker: 1 -> 0) */
+} hcg_batch;
+
+/* Per-path record, PathResult fields (src/path_result.jl:76-98). */
+typedef struct hcg_path_result {
+    int32_t code;              /* EndgameTrackerCode ordinal */
+    int32_t singular;
+    int32_t winding_number;    /* 0 = nothing */
+    int32_t extended_precision;
+    int32_t extended_precision_used;
+    int32_t accepted_steps;
+    int32_t rejected_steps;
+    int32_t steps_eg;
+    int32_t valuation_valid;   /* 1 when t <= endgame_start (valuation reported) */
+    int32_t newton_iters;      /* GPU extras */
+    int32_t learned_proposed;
+    int32_t learned_accepted;
+    double t, accuracy, residual, cond, omega, mu, last_t;
+} hcg_path_result;
+
+/* Optional per-step feature log (training data for the learned proposer). */
+typedef struct hcg_step_log {
+    int32_t capacity;   /* max records */
+    int32_t pad_;
+    float* records;     /* capacity * HCG_STEP_RECORD_FLOATS */
+    int32_t* count;     /* out: records written (may exceed capacity; extra are dropped) */
+} hcg_step_log;
+#define HCG_STEP_RECORD_FLOATS 12
+
+int hcg_ctx_create(int device, hcg_ctx** out);
+void hcg_ctx_destroy(hcg_ctx* ctx);
+
+/* Generates HIP source for the homotopy's tapes, compiles it with hipRTC for gfx950
+ * (cached by source hash) and loads the tracker kernels.  Counterpart of
+ * CompiledSystem/CompiledHomotopy code generation
+ * (src/model_kit/compiled_system_homotopy.jl:15-58, 178-361). */
+int hcg_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* h, hcg_program** out);
+void hcg_program_destroy(hcg_program* prog);
+/* Generated HIP source of a program (for inspection); valid until destroy. */
+const char* hcg_program_source(const hcg_program* prog);
+
+void hcg_default_options(hcg_options* opts);
+
+/* Track every path of `batch` to completion in one persistent launch and write one
+ * hcg_path_result per path plus solution / last_point (npaths*n complex) and
+ * valuation (npaths*n doubles).  Replaces threaded_solve's loop of
+ * track(tracker, S[k]; path_number = k) (src/solve.jl:628-709). */
+int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch,
+                    const hcg_options* opts, hcg_path_result* results, double* solutions,
+                    double* last_points, double* valuations, hcg_step_log* log);
+
+/* Time of the last hcg_track_batch's tracker kernel (ms, HIP events on its stream),
+ * and the device-side counters of that launch. */
+int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps,
+                          int64_t* total_newton_iters);
+
+/* Learned step-size proposer: 8 features -> 64 -> 64 (GELU) -> 2 outputs
+ * (log step, at-infinity logit).  Weights are BF16 with FP32 accumulation. */
+#define HCG_MLP_IN 8
+#define HCG_MLP_HIDDEN 64
+#define HCG_MLP_OUT 2
+#define HCG_MLP_NPARAMS (HCG_MLP_IN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_OUT + HCG_MLP_OUT)
+/* Load FP32 master weights (HCG_MLP_NPARAMS floats, layout W1[64][8] b1 W2[64][64] b2 W3[2][64] b3)
+ * into the context; the tracker rounds them to BF16. */
+int hcg_mlp_load(hcg_ctx* ctx, const float* params);
+/* Forward + backward on the device for one batch: writes loss and gradients
+ * (HCG_MLP_NPARAMS floats).  x: n*8, y: n*2 targets, w: n*2 per-output weights. */
+int hcg_mlp_forward_backward(hcg_ctx* ctx, const float* params, int32_t n, const float* x,
+                             const float* y, const float* w, float* out_pred, float* loss,
+                             float* grad);
+/* Train with Adam for `epochs` over (x, y, w) on the device; params in/out. */
+int hcg_mlp_train(hcg_ctx* ctx, float* params, int32_t n, const float* x, const float* y,
+                  const float* w, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                  float* final_loss);
+
+const char* hcg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
