@@ -1,0 +1,534 @@
+// Tape -> HIP source generator (host side of hcg_program_create).
+//
+// Lowers each straight-line program (ModelKit InstructionSequence layout,
+// src/model_kit/instruction_sequence.jl:1-27) to straight-line device functions:
+//   *_eval      u = F(x)                    (compiled_execute_impl, compiled_system_homotopy.jl:178-243)
+//   *_eval_jac  u = F(x), U = dF/dx         (evaluate_and_jacobian!, :277-289)
+//   *_eval_dd   F(x) in double-double       (interpreted_system.jl:65-70)
+//   *_taylorK   orders K-1 and K of F(x(s)) (execute_taylor!, instruction_interpreter.jl:444-491)
+// Taylor code is emitted per coefficient: values carry the highest order that can be
+// non-zero (constants 0, parameters 0 or 1, variables K-1), and products only emit
+// the terms that are not structurally zero.  The accumulation order of every
+// remaining term is the oracle's, so the results agree bit for bit apart from the
+// sign of exact zeros.
+#include "hcg_codegen.h"
+
+#include <array>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace hcg {
+
+enum {
+    OP_STOP = 0, OP_CB, OP_ACOS, OP_ASIN, OP_COS, OP_COSH, OP_EXP, OP_INV, OP_INV_NOT_ZERO, OP_INVSQR, OP_NEG,
+    OP_SIN, OP_SINH, OP_SQR, OP_SQRT, OP_TAN, OP_TANH, OP_IDENTITY, OP_ADD, OP_DIV, OP_MUL, OP_SUB, OP_POW_INT,
+    OP_POW, OP_ADD3, OP_MUL3, OP_MULADD, OP_MULSUB, OP_SUBMUL, OP_ADD4, OP_MUL4, OP_MULMULADD, OP_MULMULSUB
+};
+
+static int arity(int op) {
+    switch (op) {
+    case OP_STOP: return 0;
+    case OP_CB: case OP_INV: case OP_INV_NOT_ZERO: case OP_INVSQR: case OP_NEG: case OP_SQR: case OP_IDENTITY: return 1;
+    case OP_ADD: case OP_DIV: case OP_MUL: case OP_SUB: case OP_POW_INT: return 2;
+    case OP_ADD3: case OP_MUL3: case OP_MULADD: case OP_MULSUB: case OP_SUBMUL: return 3;
+    case OP_ADD4: case OP_MUL4: case OP_MULMULADD: case OP_MULMULSUB: return 4;
+    default: return -1;
+    }
+}
+
+static std::string dlit(double v) {
+    if (std::isnan(v)) return "__builtin_nan(\"\")";
+    if (std::isinf(v)) return v > 0 ? "__builtin_inf()" : "(-__builtin_inf())";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%a", v);
+    return buf;
+}
+static std::string clit(double re, double im) { return "mkc(" + dlit(re) + ", " + dlit(im) + ")"; }
+
+bool validate_tape(const hcg_tape* tp, int n, std::string* err) {
+    if (!tp || !tp->instr) { *err = "null tape"; return false; }
+    if (tp->t_index != 0) { *err = "homotopy tapes (continuation parameter in the tape) are not supported"; return false; }
+    if (tp->nvars != n) { *err = "tape variable count differs from the homotopy size"; return false; }
+    if (tp->output_dim != n) { *err = "only square systems are supported"; return false; }
+    if (tp->nconst < 0 || tp->nparams < 0 || tp->tape_space < tp->nconst + tp->nparams + tp->nvars) { *err = "bad tape ranges"; return false; }
+    if (tp->params_first != tp->nconst + 1 || tp->vars_first != tp->nconst + tp->nparams + 1) { *err = "unexpected tape block layout"; return false; }
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        int ar = arity(op);
+        if (ar < 0) { *err = "unsupported SLP op " + std::to_string(op) + " (transcendental ops are not lowered)"; return false; }
+        if (op == OP_STOP) break;
+        for (int k = 0; k < ar; k++) {
+            if (op == OP_POW_INT && k == 1) continue;
+            if (I[k] < 1 || I[k] > tp->tape_space) { *err = "tape input index out of range"; return false; }
+        }
+        if (I[5] < 1 || I[5] > tp->tape_space) { *err = "tape output index out of range"; return false; }
+        if (I[5] <= tp->nconst + tp->nparams + tp->nvars) { *err = "instruction overwrites an input slot"; return false; }
+    }
+    for (int i = 0; i < tp->nu; i++)
+        if (tp->u_assign[2 * i] < 1 || tp->u_assign[2 * i] > n || tp->u_assign[2 * i + 1] < 1 || tp->u_assign[2 * i + 1] > tp->tape_space) { *err = "bad u assignment"; return false; }
+    for (int i = 0; i < tp->nU; i++)
+        if (tp->U_assign[2 * i] < 1 || tp->U_assign[2 * i] > n * n || tp->U_assign[2 * i + 1] < 1 || tp->U_assign[2 * i + 1] > tp->tape_space) { *err = "bad U assignment"; return false; }
+    return true;
+}
+
+enum ParamMode { PM_NONE, PM_FIXED, PM_DYNAMIC };
+
+// ---------------------------- ComplexF64 / DD eval ----------------------------
+static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, bool jac, bool dd, int n,
+                     ParamMode pm, const double* fixed, long* flops) {
+    const char* T = dd ? "cdd" : "cplx";
+    os << "__device__ __noinline__ void " << fname << "(const " << T << "* __restrict__ x, ";
+    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, ";
+    os << (dd ? "cdd" : "cplx") << "* __restrict__ u";
+    if (jac) os << ", cplx* __restrict__ U";
+    os << ") {\n";
+    std::vector<std::string> nm(tp->tape_space + 1);
+    auto wrap = [&](const std::string& c) { return dd ? "cdd_from(" + c + ")" : c; };
+    for (int k = 0; k < tp->nconst; k++)
+        nm[k + 1] = wrap(clit(tp->constants[2 * k], tp->constants[2 * k + 1]));
+    for (int k = 0; k < tp->nparams; k++) {
+        std::string v = "P" + std::to_string(k);
+        if (pm == PM_FIXED) os << "  const " << T << " " << v << " = " << wrap(clit(fixed[2 * k], fixed[2 * k + 1])) << ";\n";
+        else os << "  const " << T << " " << v << " = " << wrap("p[" + std::to_string(k) + "]") << ";\n";
+        nm[tp->params_first + k] = v;
+    }
+    for (int k = 0; k < tp->nvars; k++) {
+        std::string v = "X" + std::to_string(k);
+        os << "  const " << T << " " << v << " = x[" << k << "];\n";
+        nm[tp->vars_first + k] = v;
+    }
+    const char* pre = dd ? "cdd_" : "c_";
+    long f = 0;
+    long dm = dd ? 58 : 6, da = dd ? 22 : 2;
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        if (op == OP_STOP) break;
+        std::string a = nm[I[0]], b = arity(op) > 1 && op != OP_POW_INT ? nm[I[1]] : "", c = arity(op) > 2 ? nm[I[2]] : "",
+                    d = arity(op) > 3 ? nm[I[3]] : "";
+        std::string out = "T" + std::to_string(i);
+        std::string e;
+        switch (op) {
+        case OP_CB: e = std::string(pre) + "cb(" + a + ")"; f += dm * 2; break;
+        case OP_INV: e = std::string(pre) + "inv(" + a + ")"; f += dm + da; break;
+        case OP_INV_NOT_ZERO: e = std::string(pre) + "inv_not_zero(" + a + ")"; f += dm + da; break;
+        case OP_INVSQR: e = std::string(pre) + "invsqr(" + a + ")"; f += 2 * dm + da; break;
+        case OP_NEG: e = std::string(pre) + "neg(" + a + ")"; break;
+        case OP_SQR: e = std::string(pre) + "sqr(" + a + ")"; f += dd ? 3 * 11 + 2 * 9 : 5; break;
+        case OP_IDENTITY: e = a; break;
+        case OP_ADD: e = std::string(pre) + "add(" + a + ", " + b + ")"; f += da; break;
+        case OP_SUB: e = std::string(pre) + "sub(" + a + ", " + b + ")"; f += da; break;
+        case OP_MUL: e = std::string(pre) + "mul(" + a + ", " + b + ")"; f += dm; break;
+        case OP_DIV: e = std::string(pre) + "div(" + a + ", " + b + ")"; f += dm + da + 3; break;
+        case OP_POW_INT: e = std::string(pre) + "pow_int(" + a + ", " + std::to_string(I[1]) + ")"; f += dm * 2; break;
+        case OP_ADD3: e = std::string(pre) + "add3(" + a + ", " + b + ", " + c + ")"; f += 2 * da; break;
+        case OP_MUL3: e = std::string(pre) + "mul3(" + a + ", " + b + ", " + c + ")"; f += 2 * dm; break;
+        case OP_MULADD: e = std::string(pre) + "muladd(" + a + ", " + b + ", " + c + ")"; f += dm + da; break;
+        case OP_MULSUB: e = std::string(pre) + "mulsub(" + a + ", " + b + ", " + c + ")"; f += dm + da; break;
+        case OP_SUBMUL: e = std::string(pre) + "submul(" + a + ", " + b + ", " + c + ")"; f += dm + da; break;
+        case OP_ADD4: e = std::string(pre) + "add4(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 3 * da; break;
+        case OP_MUL4: e = std::string(pre) + "mul4(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 3 * dm; break;
+        case OP_MULMULADD: e = std::string(pre) + "mulmuladd(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 2 * dm + da; break;
+        case OP_MULMULSUB: e = std::string(pre) + "mulmulsub(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 2 * dm + da; break;
+        }
+        os << "  const " << T << " " << out << " = " << e << ";\n";
+        nm[I[5]] = out;
+    }
+    std::vector<int> uset(n, 0);
+    for (int i = 0; i < tp->nu; i++) {
+        int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
+        os << "  u[" << row << "] = " << nm[slot] << ";\n";
+        uset[row] = 1;
+    }
+    for (int i = 0; i < n; i++)
+        if (!uset[i]) os << "  u[" << i << "] = " << (dd ? "cdd_from(mkc(0.0, 0.0))" : "mkc(0.0, 0.0)") << ";\n";
+    if (jac) {
+        std::vector<int> Uset(n * n, 0);
+        for (int i = 0; i < tp->nU; i++) {
+            int l = tp->U_assign[2 * i] - 1, slot = tp->U_assign[2 * i + 1];
+            os << "  U[" << l << "] = " << nm[slot] << ";\n";
+            Uset[l] = 1;
+        }
+        for (int i = 0; i < n * n; i++)
+            if (!Uset[i]) os << "  U[" << i << "] = mkc(0.0, 0.0);\n";
+    }
+    os << "}\n";
+    *flops = f;
+}
+
+// ------------------------ double-double tape interpreter ------------------------
+// The double-double residual runs rarely (extended-precision steps and final
+// refinement), so it is interpreted like the reference's InterpretedSystem
+// (src/model_kit/interpreted_system.jl:65-70, instruction_interpreter.jl:252-332)
+// instead of being unrolled: the tape is re-encoded with its slots compacted by a
+// linear scan (operand >= 0: register slot, < 0: constant -k-1) and stored in
+// __constant__ tables; each lane keeps its live slots in a private array.
+static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, int n, ParamMode pm,
+                          const double* fixed, long* flops) {
+    const int base = tp->nconst + tp->nparams + tp->nvars;
+    std::vector<double> consts;
+    for (int k = 0; k < 2 * tp->nconst; k++) consts.push_back(tp->constants[k]);
+    // operand of every tape slot at the current point: >=0 register, <0 constant
+    std::vector<int> where(tp->tape_space + 1, 0);
+    std::vector<int> last_use(tp->tape_space + 1, -1);
+    int ninstr = 0;
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        if (I[4] == OP_STOP) break;
+        ninstr++;
+    }
+    // last use of each (slot, definition) approximated per slot: the instructions are SSA
+    // after validation except for slot reuse, so track uses per definition index
+    std::vector<int> def_of(tp->tape_space + 1, -1);   // current definition id of slot
+    std::vector<int> def_last(ninstr + tp->nparams + tp->nvars + 1, -1);
+    int ndefs = 0;
+    std::vector<int> input_def(tp->tape_space + 1, -1);
+    for (int k = 0; k < tp->nparams + tp->nvars; k++) {
+        int slot = tp->params_first + k;
+        def_of[slot] = ndefs++;
+    }
+    std::vector<std::array<int, 4>> in_defs(ninstr);
+    std::vector<int> out_def(ninstr);
+    for (int i = 0; i < ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4], ar = arity(op);
+        for (int q = 0; q < 4; q++) {
+            in_defs[i][q] = -2;  // -2 unused, -1 constant
+            if (q >= ar || (op == OP_POW_INT && q == 1)) continue;
+            int s = I[q];
+            if (s <= tp->nconst) { in_defs[i][q] = -1; continue; }
+            in_defs[i][q] = def_of[s];
+            def_last[def_of[s]] = i;
+        }
+        def_of[I[5]] = ndefs;
+        out_def[i] = ndefs++;
+    }
+    def_last.resize(ndefs, -1);
+    // outputs keep their definitions alive to the end
+    std::vector<int> out_operand(n, INT32_MIN);
+    std::vector<int> out_is_def(n, -1), out_const(n, -1);
+    for (int i = 0; i < tp->nu; i++) {
+        int row = tp->u_assign[2 * i] - 1, s = tp->u_assign[2 * i + 1];
+        if (s <= tp->nconst) out_const[row] = s - 1;
+        else { out_is_def[row] = def_of[s]; def_last[def_of[s]] = ninstr; }
+    }
+    // linear scan register assignment
+    std::vector<int> reg(ndefs, -1);
+    std::vector<int> free_regs;
+    int nreg = 0;
+    auto alloc = [&]() { if (!free_regs.empty()) { int r = free_regs.back(); free_regs.pop_back(); return r; } return nreg++; };
+    std::vector<std::pair<int, int>> loads;  // (register, input index) for params+vars
+    for (int k = 0; k < tp->nparams + tp->nvars; k++) {
+        if (def_last[k] < 0) continue;  // unused input
+        reg[k] = alloc();
+        loads.push_back({reg[k], k});
+    }
+    std::vector<int> code;
+    for (int i = 0; i < ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        int ops[4] = {0, 0, 0, 0};
+        for (int q = 0; q < 4; q++) {
+            int d = in_defs[i][q];
+            if (d == -2) ops[q] = (op == OP_POW_INT && q == 1) ? I[1] : 0;
+            else if (d == -1) ops[q] = -(I[q] - 1) - 1;
+            else ops[q] = reg[d];
+        }
+        // free inputs whose last use is this instruction
+        for (int q = 0; q < 4; q++) {
+            int d = in_defs[i][q];
+            if (d >= 0 && def_last[d] == i && reg[d] >= 0) {
+                bool dup = false;
+                for (int r = 0; r < q; r++) if (in_defs[i][r] == d) dup = true;
+                if (!dup) free_regs.push_back(reg[d]);
+            }
+        }
+        int od = out_def[i];
+        int outr = def_last[od] >= 0 ? alloc() : alloc();
+        reg[od] = outr;
+        if (def_last[od] < 0) free_regs.push_back(outr);  // dead value: slot reusable right away
+        code.insert(code.end(), {op, ops[0], ops[1], ops[2], ops[3], outr});
+    }
+    for (int r = 0; r < n; r++) {
+        if (out_is_def[r] >= 0) out_operand[r] = reg[out_is_def[r]];
+        else if (out_const[r] >= 0) out_operand[r] = -out_const[r] - 1;
+    }
+    // fixed parameters become constants: rewrite their register loads
+    os << "__constant__ int " << fname << "_code[] = {";
+    for (size_t k = 0; k < code.size(); k++) os << (k ? "," : "") << code[k];
+    if (code.empty()) os << "0";
+    os << "};\n__constant__ double " << fname << "_const[] = {";
+    for (size_t k = 0; k < consts.size(); k++) os << (k ? "," : "") << dlit(consts[k]);
+    if (consts.empty()) os << "0.0";
+    os << "};\n";
+    os << "__device__ __noinline__ void " << fname << "(const cdd* __restrict__ x, ";
+    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, ";
+    os << "cdd* __restrict__ u) {\n";
+    os << "  cdd R[" << std::max(nreg, 1) << "];\n";
+    for (auto& ld : loads) {
+        int k = ld.second;
+        if (k < tp->nparams) {
+            if (pm == PM_FIXED) os << "  R[" << ld.first << "] = cdd_from(" << clit(fixed[2 * k], fixed[2 * k + 1]) << ");\n";
+            else os << "  R[" << ld.first << "] = cdd_from(p[" << k << "]);\n";
+        } else {
+            os << "  R[" << ld.first << "] = x[" << (k - tp->nparams) << "];\n";
+        }
+    }
+    os << "  dd_interpret(" << fname << "_code, " << ninstr << ", " << fname << "_const, R);\n";
+    for (int r = 0; r < n; r++) {
+        if (out_operand[r] == INT32_MIN) os << "  u[" << r << "] = cdd_from(mkc(0.0, 0.0));\n";
+        else if (out_operand[r] >= 0) os << "  u[" << r << "] = R[" << out_operand[r] << "];\n";
+        else { int k = -out_operand[r] - 1; os << "  u[" << r << "] = cdd_from(" << clit(tp->constants[2 * k], tp->constants[2 * k + 1]) << ");\n"; }
+    }
+    os << "}\n";
+    long f = 0;
+    for (int i = 0; i < ninstr; i++) {
+        int op = code[6 * i];
+        switch (op) {
+        case OP_MUL: f += 58; break;
+        case OP_MULADD: case OP_MULSUB: case OP_SUBMUL: f += 80; break;
+        case OP_ADD: case OP_SUB: f += 22; break;
+        case OP_SQR: f += 51; break;
+        default: f += 120; break;
+        }
+    }
+    *flops = f;
+}
+
+// ---------------------------- Taylor series ----------------------------
+struct SVal {
+    int deg;                    // highest possibly non-zero order, -1 = identically zero
+    std::vector<std::string> c; // coefficient expressions (valid for 0..deg)
+};
+
+static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, int K, int n, ParamMode pm,
+                       const double* fixed, long* flops) {
+    os << "__device__ __noinline__ void " << fname << "(";
+    for (int j = 0; j < K; j++) os << "const cplx* __restrict__ t" << j << ", ";
+    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, const cplx* __restrict__ dp, ";
+    os << "cplx* __restrict__ oKm1, cplx* __restrict__ oK) {\n";
+    std::vector<SVal> v(tp->tape_space + 1);
+    auto zero = [&]() { SVal s; s.deg = -1; return s; };
+    for (auto& s : v) s = zero();
+    for (int k = 0; k < tp->nconst; k++) {
+        SVal s; s.deg = 0; s.c.push_back(clit(tp->constants[2 * k], tp->constants[2 * k + 1]));
+        v[k + 1] = s;
+    }
+    for (int k = 0; k < tp->nparams; k++) {
+        SVal s;
+        std::string nm0 = "P" + std::to_string(k);
+        if (pm == PM_FIXED) {
+            os << "  const cplx " << nm0 << " = " << clit(fixed[2 * k], fixed[2 * k + 1]) << ";\n";
+            s.deg = 0; s.c.push_back(nm0);
+        } else {
+            os << "  const cplx " << nm0 << " = p[" << k << "];\n";
+            std::string nm1 = "Q" + std::to_string(k);
+            os << "  const cplx " << nm1 << " = dp[" << k << "];\n";
+            s.deg = K >= 1 ? 1 : 0; s.c.push_back(nm0);
+            if (K >= 1) s.c.push_back(nm1);
+        }
+        v[tp->params_first + k] = s;
+    }
+    for (int k = 0; k < tp->nvars; k++) {
+        SVal s;
+        s.deg = K - 1;
+        for (int j = 0; j < K; j++) {
+            std::string nm = "X" + std::to_string(k) + "_" + std::to_string(j);
+            os << "  const cplx " << nm << " = t" << j << "[" << k << "];\n";
+            s.c.push_back(nm);
+        }
+        v[tp->vars_first + k] = s;
+    }
+    long f = 0;
+    auto has = [&](const SVal& s, int j) { return j >= 0 && j <= s.deg; };
+    auto pack = [&](const SVal& s) {  // full ser<K> expression
+        std::string e = "ser<" + std::to_string(K) + ">{{";
+        for (int j = 0; j <= K; j++) {
+            if (j) e += ", ";
+            e += has(s, j) ? s.c[j] : "mkc(0.0, 0.0)";
+        }
+        return e + "}}";
+    };
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        if (op == OP_STOP) break;
+        std::string base = "S" + std::to_string(i);
+        const SVal& a = v[I[0]];
+        SVal r;
+        auto emit = [&](int k, const std::string& e) {
+            std::string nm = base + "_" + std::to_string(k);
+            os << "  const cplx " << nm << " = " << e << ";\n";
+            if ((int)r.c.size() <= k) r.c.resize(k + 1);
+            r.c[k] = nm;
+        };
+        auto prod_terms = [&](const SVal& x, const SVal& y, int k, std::string acc, bool have_acc) {
+            // Σ_{j} x_j y_{k-j}, j ascending, skipping structural zeros; muladd chain on acc
+            for (int j = 0; j <= k; j++) {
+                if (!has(x, j) || !has(y, k - j)) continue;
+                if (!have_acc) { acc = "c_mul(" + x.c[j] + ", " + y.c[k - j] + ")"; have_acc = true; f += 6; }
+                else { acc = "c_muladd(" + x.c[j] + ", " + y.c[k - j] + ", " + acc + ")"; f += 8; }
+            }
+            return std::make_pair(acc, have_acc);
+        };
+        switch (op) {
+        case OP_IDENTITY: r = a; break;
+        case OP_NEG:
+            r.deg = a.deg;
+            for (int k = 0; k <= a.deg; k++) emit(k, "c_neg(" + a.c[k] + ")");
+            break;
+        case OP_ADD:
+        case OP_SUB: {
+            const SVal& b = v[I[1]];
+            r.deg = std::max(a.deg, b.deg);
+            for (int k = 0; k <= r.deg; k++) {
+                bool ha = has(a, k), hb = has(b, k);
+                if (ha && hb) { emit(k, std::string(op == OP_ADD ? "c_add(" : "c_sub(") + a.c[k] + ", " + b.c[k] + ")"); f += 2; }
+                else if (ha) emit(k, a.c[k]);
+                else if (hb) emit(k, op == OP_ADD ? b.c[k] : "c_sub(mkc(0.0, 0.0), " + b.c[k] + ")");
+                else emit(k, "mkc(0.0, 0.0)");
+            }
+            break;
+        }
+        case OP_MUL: {
+            const SVal& b = v[I[1]];
+            if (a.deg < 0 || b.deg < 0) { r.deg = -1; break; }
+            r.deg = std::min(K, a.deg + b.deg);
+            for (int k = 0; k <= r.deg; k++) {
+                auto pr = prod_terms(a, b, k, "", false);
+                emit(k, pr.second ? pr.first : "mkc(0.0, 0.0)");
+            }
+            break;
+        }
+        case OP_MULADD: {
+            const SVal& b = v[I[1]];
+            const SVal& c = v[I[2]];
+            int dp = (a.deg < 0 || b.deg < 0) ? -1 : std::min(K, a.deg + b.deg);
+            r.deg = std::max(dp, c.deg);
+            for (int k = 0; k <= r.deg; k++) {
+                bool hc = has(c, k);
+                auto pr = prod_terms(a, b, k, hc ? c.c[k] : "", hc);
+                emit(k, pr.second ? pr.first : "mkc(0.0, 0.0)");
+            }
+            break;
+        }
+        case OP_SQR: {
+            if (a.deg < 0) { r.deg = -1; break; }
+            r.deg = std::min(K, 2 * a.deg);
+            for (int k = 0; k <= r.deg; k++) {
+                if (k == 0) { emit(0, "c_sqr(" + a.c[0] + ")"); f += 5; continue; }
+                std::string w; bool hw = false;
+                for (int j = 0; j <= (k - 1) / 2; j++) {
+                    if (!has(a, j) || !has(a, k - j)) continue;
+                    if (!hw) { w = "c_mul(" + a.c[j] + ", " + a.c[k - j] + ")"; hw = true; f += 6; }
+                    else { w = "c_muladd(" + a.c[j] + ", " + a.c[k - j] + ", " + w + ")"; f += 8; }
+                }
+                if (k % 2 == 0) {
+                    bool hs = has(a, k / 2);
+                    std::string sq = hs ? "c_sqr(" + a.c[k / 2] + ")" : "mkc(0.0, 0.0)";
+                    if (hw) { emit(k, "c_add(c_scale(2.0, " + w + "), " + sq + ")"); f += 9; }
+                    else emit(k, sq);
+                } else {
+                    if (hw) { emit(k, "c_add(" + w + ", " + w + ")"); f += 2; }
+                    else emit(k, "mkc(0.0, 0.0)");
+                }
+            }
+            break;
+        }
+        default: {
+            // remaining ops: full-series templates
+            std::string args = pack(a);
+            int ar = arity(op);
+            std::string fn;
+            switch (op) {
+            case OP_CB: fn = "s_cb"; f += 12 * (K + 1) * (K + 2); break;
+            case OP_INV: case OP_INV_NOT_ZERO: fn = "s_inv"; f += 8 * (K + 1) * (K + 2); break;
+            case OP_INVSQR: fn = "s_invsqr"; f += 12 * (K + 1) * (K + 2); break;
+            case OP_DIV: fn = "s_div"; f += 8 * (K + 1) * (K + 2); break;
+            case OP_POW_INT: fn = "s_pow_int"; f += 16 * (K + 1) * (K + 2); break;
+            case OP_ADD3: fn = "s_add3"; f += 4 * (K + 1); break;
+            case OP_MUL3: fn = "s_mul3"; f += 8 * (K + 1) * (K + 2); break;
+            case OP_MULSUB: fn = "s_mulsub"; f += 4 * (K + 1) * (K + 2); break;
+            case OP_SUBMUL: fn = "s_submul"; f += 4 * (K + 1) * (K + 2); break;
+            case OP_ADD4: fn = "s_add4"; f += 6 * (K + 1); break;
+            case OP_MUL4: fn = "s_mul4"; f += 12 * (K + 1) * (K + 2); break;
+            case OP_MULMULADD: fn = "s_mulmuladd"; f += 8 * (K + 1) * (K + 2); break;
+            case OP_MULMULSUB: fn = "s_mulmulsub"; f += 8 * (K + 1) * (K + 2); break;
+            }
+            if (op == OP_POW_INT) args += ", " + std::to_string(I[1]);
+            else for (int q = 1; q < ar; q++) args += ", " + pack(v[I[q]]);
+            os << "  const ser<" << K << "> " << base << " = " << fn << "<" << K << ">(" << args << ");\n";
+            r.deg = K;
+            r.c.resize(K + 1);
+            for (int k = 0; k <= K; k++) r.c[k] = base + ".c[" + std::to_string(k) + "]";
+            break;
+        }
+        }
+        v[I[5]] = r;
+    }
+    std::vector<int> uset(n, 0);
+    for (int i = 0; i < tp->nu; i++) {
+        int row = tp->u_assign[2 * i] - 1;
+        const SVal& s = v[tp->u_assign[2 * i + 1]];
+        os << "  oKm1[" << row << "] = " << (has(s, K - 1) ? s.c[K - 1] : "mkc(0.0, 0.0)") << ";\n";
+        os << "  oK[" << row << "] = " << (has(s, K) ? s.c[K] : "mkc(0.0, 0.0)") << ";\n";
+        uset[row] = 1;
+    }
+    for (int i = 0; i < n; i++)
+        if (!uset[i]) os << "  oKm1[" << i << "] = mkc(0.0, 0.0);\n  oK[" << i << "] = mkc(0.0, 0.0);\n";
+    os << "}\n";
+    *flops = f;
+}
+
+static void gen_system(std::ostringstream& os, const hcg_system& s, const std::string& prefix, int n, ParamMode pm,
+                       bool straight, Flops* fl) {
+    const double* fixed = s.fixed_params;
+    long f;
+    gen_eval(os, s.eval, prefix + "_eval", false, false, n, pm, fixed, &f); fl->eval += f;
+    gen_eval(os, s.eval_jac, prefix + "_eval_jac", true, false, n, pm, fixed, &f); fl->eval_jac += f;
+    gen_dd_interp(os, s.eval, prefix + "_eval_dd", n, pm, fixed, &f); fl->eval_dd += f;
+    for (int K = straight ? 2 : 1; K <= 3; K++) {
+        gen_taylor(os, s.eval, prefix + "_taylor" + std::to_string(K), K, n, pm, fixed, &f);
+        fl->taylor[K] += f;
+    }
+}
+
+bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, std::string* err) {
+    int n = h->n;
+    if (n < 1 || n > HCG_MAX_N) { *err = "system size out of range (1.." + std::to_string(HCG_MAX_N) + ")"; return false; }
+    if (h->kind != HCG_STRAIGHT_LINE && h->kind != HCG_PARAMETER) { *err = "unknown homotopy kind"; return false; }
+    if (!validate_tape(h->target.eval_jac, n, err) || !validate_tape(h->target.eval, n, err)) return false;
+    if (h->target.eval_jac->nU == 0 && n > 0) { *err = "eval_jac tape has no Jacobian assignments"; return false; }
+    if (h->target.eval_jac->nparams != h->target.eval->nparams) { *err = "eval/eval_jac parameter counts differ"; return false; }
+    if (h->kind == HCG_STRAIGHT_LINE) {
+        if (!validate_tape(h->start.eval_jac, n, err) || !validate_tape(h->start.eval, n, err)) return false;
+        if (h->start.eval->nparams && !h->start.fixed_params) { *err = "start system parameters must be fixed"; return false; }
+        if (h->target.eval->nparams && !h->target.fixed_params) { *err = "target system parameters must be fixed"; return false; }
+    } else if (h->target.eval->nparams < 1) {
+        *err = "parameter homotopy needs a system with parameters";
+        return false;
+    }
+    *fl = Flops();
+    std::ostringstream os;
+    os << "// generated by hcgpu codegen\n";
+    os << "#define HCG_N " << n << "\n#define HCG_KIND " << h->kind << "\n";
+    os << "#define HCG_NPAR " << (h->kind == HCG_PARAMETER ? h->target.eval->nparams : 0) << "\n";
+    os << "#define HCG_GAMMA_RE " << dlit(h->gamma_re) << "\n#define HCG_GAMMA_IM " << dlit(h->gamma_im) << "\n";
+    os << "#include \"hcg_arith.h\"\n";
+    if (h->kind == HCG_STRAIGHT_LINE) {
+        gen_system(os, h->start, "sysG", n, h->start.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
+        gen_system(os, h->target, "sysF", n, h->target.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
+    } else {
+        gen_system(os, h->target, "sysF", n, PM_DYNAMIC, false, fl);
+    }
+    os << "#include \"hcg_tracker.h\"\n";
+    *src = os.str();
+    return true;
+}
+
+}  // namespace hcg

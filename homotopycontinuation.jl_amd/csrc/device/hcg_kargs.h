@@ -1,0 +1,24 @@
+// Kernel argument block of hcg_track_kernel (shared by the host runtime and the
+// hipRTC-compiled device code; plain C types only).
+#pragma once
+#include "hcgpu.h"
+
+typedef struct hcg_kargs {
+    int32_t npaths;
+    int32_t log_capacity;
+    const double* starts;          // npaths * n complex
+    double t1, t0;
+    const double* start_params;    // nparams complex (PARAMETER)
+    const double* target_params;   // ngroups * nparams complex
+    const int32_t* group_of_path;  // npaths or NULL
+    int32_t* next_path;            // device work-queue head (zeroed before launch)
+    const hcg_options* opts;       // device copy
+    hcg_path_result* results;
+    double* solutions;
+    double* last_points;
+    double* valuations;
+    unsigned long long* counters;  // [0] tracker steps, [1] Newton iterations, [2] learned proposals, [3] accepted
+    float* log_records;            // step log (HCG_STEP_RECORD_FLOATS per record) or NULL
+    int32_t* log_count;
+    const float* mlp;              // learned proposer weights (FP32 master copy) or NULL
+} hcg_kargs;

@@ -1,0 +1,452 @@
+// libhcgpu.so — C ABI of the MI355X batched path tracker (see include/hcgpu.h).
+//
+// hcg_program_create: generates HIP source from the homotopy's tapes (codegen.cpp),
+//   compiles it with hipRTC for gfx950 (cached per process by source hash) and loads
+//   the persistent tracker kernel.
+// hcg_track_batch: copies a batch of start solutions to HBM, launches the persistent
+//   kernel once (lanes pull paths from a device work queue) and copies the per-path
+//   records back.  Counterpart of threaded_solve (src/solve.jl:628-709) for one batch.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <fstream>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hcg_codegen.h"
+#include "hcg_mlp.h"
+#include "hcgpu.h"
+
+#include "_gen/device_headers.inc"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(HCG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+struct hcg_ctx {
+    int device = 0;
+    int ncu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    unsigned long long last_counters[4] = {0, 0, 0, 0};
+    float* d_mlp = nullptr;
+    bool mlp_loaded = false;
+    hcg::MlpWork mlp_work;
+};
+
+struct hcg_program {
+    hcg_ctx* ctx = nullptr;
+    int n = 0, kind = 0, npar = 0;
+    std::string source;
+    hipModule_t module = nullptr;
+    hipFunction_t kernel = nullptr;
+    hcg::Flops flops;
+};
+
+extern "C" const char* hcg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hcg_ctx_create(int device, hcg_ctx** out) {
+    if (!out) return fail(HCG_ERR_INVALID_ARGUMENT, "out is NULL");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(HCG_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    hcg_ctx* c = new hcg_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) { delete c; return fail(HCG_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e)); }
+    c->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(HCG_ERR_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return HCG_OK;
+}
+
+extern "C" void hcg_ctx_destroy(hcg_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->d_mlp) hipFree(c->d_mlp);
+    hcg::mlp_work_free(&c->mlp_work);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" void hcg_default_options(hcg_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->max_steps = 10000;
+    o->extended_precision = 1;
+    o->max_step_size = INFINITY;
+    o->max_initial_step_size = INFINITY;
+    o->min_step_size = 1e-48;
+    o->min_rel_step_size = 0.0;
+    o->terminate_cond = 1e13;
+    o->a = 0.125; o->beta_a = 1.0; o->beta_omega_p = 3.0; o->beta_tau = 0.4; o->strict_beta_tau = 0.3;
+    o->min_newton_iters = 2;
+    o->max_endgame_steps = 2000;
+    o->endgame_start = 0.1;
+    o->max_endgame_extended_steps = 400;
+    o->max_winding_number = 6;
+    o->min_cond = 1e6; o->min_cond_growth = 1e4; o->min_coord_growth = 100.0;
+    o->zero_is_at_infinity = 0; o->at_infinity_check = 1; o->only_nonsingular = 0; o->refine_steps = 3;
+    o->singular_min_accuracy = 1e-6;
+    o->val_finite_tol = 0.05; o->val_at_infinity_tol = 0.01;
+    o->sing_cond = 1e14; o->sing_accuracy = 1e-12; o->scaling_threshold = -30.0;
+    o->scale_min = 1e-4; o->scale_abs_min = 1e-6; o->scale_max = std::ldexp(1.0, 511);
+}
+
+// ------------------------------ hipRTC ------------------------------
+static std::mutex g_cache_mu;
+static std::map<std::string, std::vector<char>> g_code_cache;  // source -> code object
+
+// On-disk cache of compiled code objects: <dir>/<fnv1a64(source, options, hipRTC version)>.co,
+// dir = $HCGPU_CACHE_DIR or <directory of libhcgpu.so>/rtc_cache.
+static uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
+    return h;
+}
+static std::string cache_dir() {
+    const char* env = std::getenv("HCGPU_CACHE_DIR");
+    if (env && *env) return env;
+    Dl_info info;
+    if (dladdr((void*)&fnv1a, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t k = p.rfind('/');
+        if (k != std::string::npos) return p.substr(0, k) + "/rtc_cache";
+    }
+    return "";
+}
+static bool cache_read(const std::string& path, std::vector<char>* code) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::vector<char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (buf.size() < 64) return false;
+    *code = std::move(buf);
+    return true;
+}
+static void cache_write(const std::string& dir, const std::string& path, const std::vector<char>& code) {
+    if (dir.empty()) return;
+    mkdir(dir.c_str(), 0755);
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return;
+        f.write(code.data(), (std::streamsize)code.size());
+        if (!f) return;
+    }
+    std::rename(tmp.c_str(), path.c_str());
+}
+
+static std::vector<std::string> rtc_options() {
+    std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                     "-Wno-unused-variable", "-Wno-unused-function"};
+    const char* extra = std::getenv("HCGPU_RTC_FLAGS");
+    if (extra) {
+        std::string s(extra), tok;
+        for (char ch : s) { if (ch == ' ') { if (!tok.empty()) opts.push_back(tok); tok.clear(); } else tok += ch; }
+        if (!tok.empty()) opts.push_back(tok);
+    }
+    return opts;
+}
+
+static std::string cache_key(const std::string& src) {
+    std::string k = src;
+    for (size_t h = 0; h < sizeof(k_hdr_srcs) / sizeof(k_hdr_srcs[0]); h++) k += k_hdr_srcs[h];
+    for (auto& o : rtc_options()) k += "\n" + o;
+    int maj = 0, min = 0;
+    hiprtcVersion(&maj, &min);
+    k += "\nhiprtc " + std::to_string(maj) + "." + std::to_string(min);
+    char buf[32];
+    std::snprintf(buf, sizeof buf, "%016llx", (unsigned long long)fnv1a(k));
+    return buf;
+}
+
+static int rtc_compile(const std::string& src, std::vector<char>* code) {
+    const std::string key = cache_key(src);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_code_cache.find(key);
+        if (it != g_code_cache.end()) { *code = it->second; return HCG_OK; }
+    }
+    const std::string dir = cache_dir();
+    const std::string path = dir.empty() ? "" : dir + "/" + key + ".co";
+    if (!path.empty() && cache_read(path, code)) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_code_cache[key] = *code;
+        return HCG_OK;
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hcg_program.hip", (int)(sizeof(k_hdr_names) / sizeof(k_hdr_names[0])),
+                            k_hdr_srcs, k_hdr_names) != HIPRTC_SUCCESS)
+        return fail(HCG_ERR_COMPILE, "hiprtcCreateProgram failed");
+    std::vector<std::string> optstr = rtc_options();
+    std::vector<const char*> opts;
+    for (auto& o : optstr) opts.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+    size_t logsz = 0;
+    hiprtcGetProgramLogSize(prog, &logsz);
+    std::string log(logsz + 1, '\0');
+    if (logsz) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(HCG_ERR_COMPILE, "hipRTC compile failed: %.1800s", log.c_str());
+    }
+    size_t sz = 0;
+    hiprtcGetCodeSize(prog, &sz);
+    code->resize(sz);
+    hiprtcGetCode(prog, code->data());
+    hiprtcDestroyProgram(&prog);
+    cache_write(dir, path, *code);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_code_cache[key] = *code;
+    return HCG_OK;
+}
+
+// Compile-only entry (no device needed): used by the build check and CPU tests.
+extern "C" int hcg_compile_only(const hcg_homotopy_desc* h, size_t* code_size) {
+    std::string src, err;
+    hcg::Flops fl;
+    if (!hcg::generate_source(h, &src, &fl, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    std::vector<char> code;
+    int rc = rtc_compile(src, &code);
+    if (rc) return rc;
+    if (code_size) *code_size = code.size();
+    return HCG_OK;
+}
+
+// Generated source without compiling (no device needed).
+extern "C" int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6) {
+    std::string src, err;
+    hcg::Flops fl;
+    if (!hcg::generate_source(h, &src, &fl, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    if (needed) *needed = src.size() + 1;
+    if (buf && bufsz > 0) { std::strncpy(buf, src.c_str(), bufsz - 1); buf[bufsz - 1] = 0; }
+    if (flops6) { flops6[0] = fl.eval; flops6[1] = fl.eval_jac; flops6[2] = fl.eval_dd; flops6[3] = fl.taylor[1]; flops6[4] = fl.taylor[2]; flops6[5] = fl.taylor[3]; }
+    return HCG_OK;
+}
+
+extern "C" int hcg_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* h, hcg_program** out) {
+    if (!ctx || !h || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    hcg_program* p = new hcg_program();
+    p->ctx = ctx;
+    p->n = h->n;
+    p->kind = h->kind;
+    p->npar = h->kind == HCG_PARAMETER ? h->target.eval->nparams : 0;
+    std::string err;
+    if (!hcg::generate_source(h, &p->source, &p->flops, &err)) { delete p; return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str()); }
+    std::vector<char> code;
+    int rc = rtc_compile(p->source, &code);
+    if (rc) { delete p; return rc; }
+    hipError_t e = hipModuleLoadData(&p->module, code.data());
+    if (e != hipSuccess) { delete p; return fail(HCG_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e)); }
+    e = hipModuleGetFunction(&p->kernel, p->module, "hcg_track_kernel");
+    if (e != hipSuccess) { hipModuleUnload(p->module); delete p; return fail(HCG_ERR_HIP, "hipModuleGetFunction: %s", hipGetErrorString(e)); }
+    *out = p;
+    return HCG_OK;
+}
+
+extern "C" void hcg_program_destroy(hcg_program* p) {
+    if (!p) return;
+    if (p->module) { hipSetDevice(p->ctx->device); hipModuleUnload(p->module); }
+    delete p;
+}
+
+extern "C" const char* hcg_program_source(const hcg_program* p) { return p ? p->source.c_str() : ""; }
+
+// ------------------------------ tracking ------------------------------
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+}  // namespace
+
+#include "device/hcg_kargs.h"
+
+extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                               hcg_path_result* results, double* solutions, double* last_points, double* valuations,
+                               hcg_step_log* log) {
+    if (!ctx || !prog || !b || !opts || !results || !solutions || !last_points || !valuations)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
+    const int n = prog->n;
+    const long np = b->npaths;
+    if (np < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "negative path count");
+    if (np == 0) return HCG_OK;
+    if (np > (1L << 30) / n) return fail(HCG_ERR_INVALID_ARGUMENT, "batch too large");
+    if (!b->starts) return fail(HCG_ERR_INVALID_ARGUMENT, "starts is NULL");
+    if (prog->kind == HCG_PARAMETER) {
+        if (!b->start_params || !b->target_params || b->ngroups < 1)
+            return fail(HCG_ERR_INVALID_ARGUMENT, "parameter homotopy needs start/target parameters and ngroups >= 1");
+        if (b->group_of_path)
+            for (long k = 0; k < np; k++)
+                if (b->group_of_path[k] < 0 || b->group_of_path[k] >= b->ngroups)
+                    return fail(HCG_ERR_INVALID_ARGUMENT, "group_of_path[%ld] = %d out of range", k, b->group_of_path[k]);
+    }
+    if (opts->max_winding_number < 1 || opts->min_newton_iters < 0 || opts->min_newton_iters > 20)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "invalid options");
+    if (opts->learned_step && !ctx->mlp_loaded) return fail(HCG_ERR_INVALID_ARGUMENT, "learned_step requested but no MLP loaded");
+    HIPCHK(hipSetDevice(ctx->device));
+
+    const size_t sz_starts = sizeof(double) * 2 * n * np;
+    DevBuf d_starts, d_sp, d_tp, d_grp, d_next, d_opts, d_res, d_sol, d_last, d_val, d_cnt, d_log, d_logc;
+    HIPCHK(hipMalloc(&d_starts.p, sz_starts));
+    HIPCHK(hipMalloc(&d_next.p, sizeof(int32_t)));
+    HIPCHK(hipMalloc(&d_opts.p, sizeof(hcg_options)));
+    HIPCHK(hipMalloc(&d_res.p, sizeof(hcg_path_result) * np));
+    HIPCHK(hipMalloc(&d_sol.p, sz_starts));
+    HIPCHK(hipMalloc(&d_last.p, sz_starts));
+    HIPCHK(hipMalloc(&d_val.p, sizeof(double) * n * np));
+    HIPCHK(hipMalloc(&d_cnt.p, sizeof(unsigned long long) * 4));
+    HIPCHK(hipMemcpyAsync(d_starts.p, b->starts, sz_starts, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_opts.p, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(d_next.p, 0, sizeof(int32_t), ctx->stream));
+    HIPCHK(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long) * 4, ctx->stream));
+    if (prog->kind == HCG_PARAMETER) {
+        HIPCHK(hipMalloc(&d_sp.p, sizeof(double) * 2 * prog->npar));
+        HIPCHK(hipMalloc(&d_tp.p, sizeof(double) * 2 * prog->npar * b->ngroups));
+        HIPCHK(hipMemcpyAsync(d_sp.p, b->start_params, sizeof(double) * 2 * prog->npar, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_tp.p, b->target_params, sizeof(double) * 2 * prog->npar * b->ngroups, hipMemcpyHostToDevice, ctx->stream));
+        if (b->group_of_path) {
+            HIPCHK(hipMalloc(&d_grp.p, sizeof(int32_t) * np));
+            HIPCHK(hipMemcpyAsync(d_grp.p, b->group_of_path, sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    int logcap = 0;
+    if (log && opts->log_steps) {
+        if (log->capacity < 0 || !log->records || !log->count) return fail(HCG_ERR_INVALID_ARGUMENT, "bad step log");
+        logcap = log->capacity;
+        HIPCHK(hipMalloc(&d_log.p, sizeof(float) * HCG_STEP_RECORD_FLOATS * (size_t)(logcap > 0 ? logcap : 1)));
+        HIPCHK(hipMalloc(&d_logc.p, sizeof(int32_t)));
+        HIPCHK(hipMemsetAsync(d_logc.p, 0, sizeof(int32_t), ctx->stream));
+    }
+
+    hcg_kargs ka;
+    std::memset(&ka, 0, sizeof ka);
+    ka.npaths = (int32_t)np;
+    ka.log_capacity = logcap;
+    ka.starts = (const double*)d_starts.p;
+    ka.t1 = b->t1;
+    ka.t0 = b->t0;
+    ka.start_params = (const double*)d_sp.p;
+    ka.target_params = (const double*)d_tp.p;
+    ka.group_of_path = (const int32_t*)d_grp.p;
+    ka.next_path = (int32_t*)d_next.p;
+    ka.opts = (const hcg_options*)d_opts.p;
+    ka.results = (hcg_path_result*)d_res.p;
+    ka.solutions = (double*)d_sol.p;
+    ka.last_points = (double*)d_last.p;
+    ka.valuations = (double*)d_val.p;
+    ka.counters = (unsigned long long*)d_cnt.p;
+    ka.log_records = (float*)d_log.p;
+    ka.log_count = (int32_t*)d_logc.p;
+    ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
+
+    const int block = 64;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
+    long waves_needed = (np + block - 1) / block;
+    long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
+    if (grid < 1) grid = 1;
+    void* args[] = {&ka};
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(results, d_res.p, sizeof(hcg_path_result) * np, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(solutions, d_sol.p, sz_starts, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(last_points, d_last.p, sz_starts, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(valuations, d_val.p, sizeof(double) * n * np, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->last_counters, d_cnt.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (logcap >= 0 && d_logc.p) {
+        int32_t cnt = 0;
+        HIPCHK(hipMemcpyAsync(&cnt, d_logc.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *log->count = cnt;
+        int ncopy = std::min(cnt, logcap);
+        if (ncopy > 0)
+            HIPCHK(hipMemcpy(log->records, d_log.p, sizeof(float) * HCG_STEP_RECORD_FLOATS * ncopy, hipMemcpyDeviceToHost));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    return HCG_OK;
+}
+
+extern "C" int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps, int64_t* total_newton_iters) {
+    if (!ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL ctx");
+    if (kernel_ms) *kernel_ms = ctx->last_ms;
+    if (total_steps) *total_steps = (int64_t)ctx->last_counters[0];
+    if (total_newton_iters) *total_newton_iters = (int64_t)ctx->last_counters[1];
+    return HCG_OK;
+}
+
+// Extra counters of the last launch: [0] steps, [1] Newton iterations, [2] FP64 flops (counted), [3] learned proposals accepted.
+extern "C" int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4) {
+    if (!ctx || !out4) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int i = 0; i < 4; i++) out4[i] = ctx->last_counters[i];
+    return HCG_OK;
+}
+
+// ------------------------------ learned proposer ------------------------------
+extern "C" int hcg_mlp_load(hcg_ctx* ctx, const float* params) {
+    if (!ctx || !params) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->d_mlp) HIPCHK(hipMalloc(&ctx->d_mlp, sizeof(float) * HCG_MLP_NPARAMS));
+    HIPCHK(hipMemcpy(ctx->d_mlp, params, sizeof(float) * HCG_MLP_NPARAMS, hipMemcpyHostToDevice));
+    ctx->mlp_loaded = true;
+    return HCG_OK;
+}
+
+extern "C" int hcg_mlp_forward_backward(hcg_ctx* ctx, const float* params, int32_t n, const float* x, const float* y,
+                                        const float* w, float* out_pred, float* loss, float* grad) {
+    if (!ctx || !params || !x || !y || !w || n < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::string err;
+    int rc = hcg::mlp_forward_backward(&ctx->mlp_work, ctx->stream, params, n, x, y, w, out_pred, loss, grad, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    return HCG_OK;
+}
+
+extern "C" int hcg_mlp_train(hcg_ctx* ctx, float* params, int32_t n, const float* x, const float* y, const float* w,
+                             int32_t epochs, int32_t batch, float lr, uint64_t seed, float* final_loss) {
+    if (!ctx || !params || !x || !y || !w || n < 1 || epochs < 0 || batch < 1) return fail(HCG_ERR_INVALID_ARGUMENT, "bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::string err;
+    int rc = hcg::mlp_train(&ctx->mlp_work, ctx->stream, params, n, x, y, w, epochs, batch, lr, seed, final_loss, &err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    return HCG_OK;
+}

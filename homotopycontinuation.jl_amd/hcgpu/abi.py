@@ -16,7 +16,7 @@ LIB_PATH = PKG_ROOT / "_build" / "libhcgpu.so"
 
 HCG_STRAIGHT_LINE = 0
 HCG_PARAMETER = 1
-HCG_STEP_RECORD_FLOATS = 12
+HCG_STEP_RECORD_FLOATS = 16
 HCG_MLP_IN, HCG_MLP_HIDDEN, HCG_MLP_OUT = 8, 64, 2
 HCG_MLP_NPARAMS = (HCG_MLP_IN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_HIDDEN
                    + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_OUT + HCG_MLP_OUT)
@@ -184,6 +184,10 @@ def lib():
     L.hcg_mlp_train.argtypes = [ctypes.c_void_p, P(ctypes.c_float), _I, P(ctypes.c_float), P(ctypes.c_float),
                                 P(ctypes.c_float), _I, _I, ctypes.c_float, ctypes.c_uint64, P(ctypes.c_float)]
     L.hcg_last_error.restype = ctypes.c_char_p
+    L.hcg_compile_only.argtypes = [P(hcg_homotopy_desc), P(ctypes.c_size_t)]
+    L.hcg_generate.argtypes = [P(hcg_homotopy_desc), ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t),
+                               P(ctypes.c_long)]
+    L.hcg_last_counters.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
     _lib = L
     return L
 
@@ -197,5 +201,6 @@ def check(rc: int):
 HCGPU_SYMBOLS = [
     "hcg_ctx_create", "hcg_ctx_destroy", "hcg_program_create", "hcg_program_destroy", "hcg_program_source",
     "hcg_default_options", "hcg_track_batch", "hcg_last_kernel_stats", "hcg_mlp_load",
-    "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error",
+    "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error", "hcg_compile_only", "hcg_generate",
+    "hcg_last_counters",
 ]

@@ -1,0 +1,88 @@
+"""Seeded synthetic workloads (the benchmark configurations).
+
+dense_system(n, d, seed): n dense polynomials with all monomials of degree <= d,
+    iid complex Gaussian coefficients (re, im ~ N(0, 1/2), Julia's randn(ComplexF64)).
+sparse_param_system(n, nterms, nparams, seed): n polynomials of degree <= 2 with
+    `nterms` monomials each (always the constant and x_i^2 in equation i), whose
+    coefficients are affine in `nparams` complex parameters.
+shared_factor_system(n, seed): f_i = l(x) q_i(x) + r_i(x) with one shared random
+    linear form l, random quadratic forms q_i and dense r_i of degree <= 2, so the
+    cubic parts share a factor and many total-degree paths diverge.
+"""
+from __future__ import annotations
+
+import itertools
+from typing import List, Tuple
+
+import numpy as np
+
+from .polys import Poly, System, variables
+
+
+def cgauss(rng, *shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)
+
+
+def monomials(n: int, d: int, exact: bool = False) -> List[Tuple[int, ...]]:
+    out = []
+    for deg in range(d + 1):
+        if exact and deg != d:
+            continue
+        for c in itertools.combinations_with_replacement(range(n), deg):
+            e = [0] * n
+            for i in c:
+                e[i] += 1
+            out.append(tuple(e))
+    return out
+
+
+def dense_system(n: int, d: int = 3, seed: int = 0) -> System:
+    rng = np.random.default_rng(seed)
+    mons = monomials(n, d)
+    polys = []
+    for i in range(n):
+        c = cgauss(rng, len(mons))
+        polys.append(Poly(dict(zip(mons, c)), n))
+    return System(polys, n)
+
+
+def sparse_param_system(n: int = 10, nterms: int = 20, nparams: int = 16, seed: int = 0) -> System:
+    rng = np.random.default_rng(seed)
+    mons = monomials(n, 2)
+    nsym = n + nparams
+    polys = []
+    for i in range(n):
+        zero = (0,) * n
+        sq = tuple(2 if k == i else 0 for k in range(n))
+        others = [m for m in mons if m != zero and m != sq]
+        pick = rng.choice(len(others), size=nterms - 2, replace=False)
+        support = [zero, sq] + [others[k] for k in sorted(pick)]
+        terms = {}
+        for m in support:
+            a = cgauss(rng, nparams + 1)
+            terms[m + (0,) * nparams] = a[0]
+            for k in range(nparams):
+                e = list(m) + [0] * nparams
+                e[n + k] = 1
+                terms[tuple(e)] = a[k + 1]
+        polys.append(Poly(terms, nsym))
+    return System(polys, n, nparams)
+
+
+def shared_factor_system(n: int, seed: int = 0) -> System:
+    rng = np.random.default_rng(seed)
+    xs, _ = variables(n)
+    lin = sum((complex(c) * x for c, x in zip(cgauss(rng, n), xs)), Poly.const(0, n))
+    quad = monomials(n, 2, exact=True)
+    low = monomials(n, 2)
+    polys = []
+    for i in range(n):
+        q = Poly(dict(zip(quad, cgauss(rng, len(quad)))), n)
+        r = Poly(dict(zip(low, cgauss(rng, len(low)))), n)
+        polys.append(lin * q + r)
+    return System(polys, n)
+
+
+def parameter_draws(nparams: int, count: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return cgauss(rng, count, nparams)

@@ -1,0 +1,167 @@
+"""Host-side handles of the C ABI: GPU context, compiled programs, batch tracking."""
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import abi
+from .abi import check, hcg_batch, hcg_options, hcg_path_result, hcg_step_log, lib
+
+
+class GpuContext:
+    """One hcg_ctx (device + stream).  Not thread-safe: one per host thread."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().hcg_ctx_create(device, ctypes.byref(self._h)))
+        self.device = device
+        self._programs: Dict[int, "Program"] = {}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def program(self, homotopy) -> "Program":
+        key = id(homotopy)
+        p = self._programs.get(key)
+        if p is None:
+            p = Program(self, homotopy)
+            self._programs[key] = p
+        return p
+
+    def last_kernel_ms(self) -> float:
+        ms = ctypes.c_double()
+        check(lib().hcg_last_kernel_stats(self._h, ctypes.byref(ms), None, None))
+        return ms.value
+
+    def last_counters(self):
+        out = (ctypes.c_uint64 * 4)()
+        check(lib().hcg_last_counters(self._h, out))
+        return dict(steps=int(out[0]), newton_iters=int(out[1]), flops=int(out[2]), learned_accepted=int(out[3]))
+
+    def close(self):
+        for p in self._programs.values():
+            p.close()
+        self._programs.clear()
+        if self._h:
+            lib().hcg_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[GpuContext] = None
+_lock = threading.Lock()
+
+
+def default_context() -> GpuContext:
+    global _default_ctx
+    with _lock:
+        if _default_ctx is None:
+            _default_ctx = GpuContext(0)
+        return _default_ctx
+
+
+class Program:
+    """A homotopy lowered to HIP and compiled with hipRTC (hcg_program_create)."""
+
+    def __init__(self, ctx: GpuContext, homotopy):
+        self.ctx = ctx
+        self.homotopy = homotopy
+        self.n = homotopy.n
+        self._h = ctypes.c_void_p()
+        check(lib().hcg_program_create(ctx.handle, ctypes.byref(homotopy.desc()), ctypes.byref(self._h)))
+
+    def source(self) -> str:
+        return lib().hcg_program_source(self._h).decode()
+
+    def close(self):
+        if self._h:
+            lib().hcg_program_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class BatchOutput:
+    """Per-path arrays of one hcg_track_batch call."""
+
+    FIELDS = [f for f, _ in hcg_path_result._fields_]
+
+    def __init__(self, rec: np.ndarray, sol, last, val, log=None):
+        self.records = rec
+        for f in self.FIELDS:
+            setattr(self, f, rec[f])
+        self.singular = self.singular.astype(bool)
+        self.solution = sol
+        self.last_point = last
+        self.valuation = val
+        self.log = log
+
+    def codes(self):
+        return [abi.RETURN_CODES[c] for c in self.code]
+
+
+_REC_DTYPE = np.dtype([(f, np.int32 if t is ctypes.c_int32 else np.float64) for f, t in hcg_path_result._fields_],
+                      align=True)
+assert _REC_DTYPE.itemsize == ctypes.sizeof(hcg_path_result)
+
+
+def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, target_groups=None,
+                group_of_path=None, log_capacity: int = 0) -> BatchOutput:
+    """Track all `starts` (npaths x n complex) from t=1 to t=0 on the GPU."""
+    opts = opts or abi.default_options()
+    starts = np.ascontiguousarray(np.asarray(starts, dtype=np.complex128))
+    npaths, n = starts.shape
+    if n != program.n:
+        raise ValueError(f"start solutions have {n} coordinates, the homotopy has {program.n}")
+    b = hcg_batch()
+    b.npaths = npaths
+    b.starts = _dp(starts)
+    b.t1, b.t0 = 1.0, 0.0
+    keep = [starts]
+    H = program.homotopy
+    if getattr(H, "q", None) is not None:
+        p = np.ascontiguousarray(H.p)
+        q = np.ascontiguousarray(H.q if target_groups is None else np.asarray(target_groups, dtype=np.complex128))
+        keep += [p, q]
+        b.start_params, b.target_params = _dp(p), _dp(q)
+        b.ngroups = q.shape[0]
+        if group_of_path is not None:
+            g = np.ascontiguousarray(np.asarray(group_of_path, dtype=np.int32))
+            if g.shape != (npaths,):
+                raise ValueError("group_of_path must have one entry per path")
+            keep.append(g)
+            b.group_of_path = g.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    rec = np.zeros(npaths, dtype=_REC_DTYPE)
+    sol = np.zeros((npaths, n), dtype=np.complex128)
+    last = np.zeros((npaths, n), dtype=np.complex128)
+    val = np.zeros((npaths, n), dtype=np.float64)
+    logp = None
+    log_arr = None
+    if log_capacity > 0:
+        opts.log_steps = 1
+        log_arr = np.zeros((log_capacity, abi.HCG_STEP_RECORD_FLOATS), dtype=np.float32)
+        cnt = ctypes.c_int32(0)
+        lg = hcg_step_log()
+        lg.capacity = log_capacity
+        lg.records = log_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        lg.count = ctypes.pointer(cnt)
+        logp = ctypes.byref(lg)
+        keep += [log_arr, cnt, lg]
+    check(lib().hcg_track_batch(program.ctx.handle, program._h, ctypes.byref(b), ctypes.byref(opts),
+                                rec.ctypes.data_as(ctypes.POINTER(hcg_path_result)), _dp(sol), _dp(last), _dp(val),
+                                logp))
+    log = None
+    if log_arr is not None:
+        log = log_arr[: min(keep[-2].value, log_capacity)]
+    return BatchOutput(rec, sol, last, val, log)

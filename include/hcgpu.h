@@ -149,14 +149,19 @@ typedef struct hcg_path_result {
     double t, accuracy, residual, cond, omega, mu, last_t;
 } hcg_path_result;
 
-/* Optional per-step feature log (training data for the learned proposer). */
+/* Optional per-step record log (training data for the learned proposer; modelled on
+ * path_info's per-step table, src/path_info.jl:3-84).  One record per tracker step:
+ *  0 path index, 1 step index, 2 t, 3 log Δs, 4 accepted, 5 Newton iterations, 6 θ,
+ *  7 log(local error · Δs^4), 8 log10 cond(H_x, ẋ), 9 log ω, 10 log μ, 11 log τ,
+ *  12 log of the next proposed Δs, 13 flags (1 extended precision, 2 singular endgame,
+ *  4 learned proposal used), 14 log10 accuracy, 15 endgame steps. */
 typedef struct hcg_step_log {
     int32_t capacity;   /* max records */
     int32_t pad_;
     float* records;     /* capacity * HCG_STEP_RECORD_FLOATS */
     int32_t* count;     /* out: records written (may exceed capacity; extra are dropped) */
 } hcg_step_log;
-#define HCG_STEP_RECORD_FLOATS 12
+#define HCG_STEP_RECORD_FLOATS 16
 
 int hcg_ctx_create(int device, hcg_ctx** out);
 void hcg_ctx_destroy(hcg_ctx* ctx);

@@ -39,6 +39,9 @@ def lib():
         D = ctypes.c_double
         L.orc_track_batch.argtypes = [P(hcg_homotopy_desc), P(hcg_batch), P(hcg_options), ctypes.c_int,
                                       P(hcg_path_result), P(D), P(D), P(D)]
+        L.orc_track_batch_log.argtypes = [P(hcg_homotopy_desc), P(hcg_batch), P(hcg_options), ctypes.c_int,
+                                          P(hcg_path_result), P(D), P(D), P(D), P(ctypes.c_float), ctypes.c_int,
+                                          P(ctypes.c_int)]
         L.orc_track_plain.argtypes = [P(hcg_homotopy_desc), P(hcg_options), P(D), P(D), P(D), D, D, P(D),
                                       P(ctypes.c_int32), P(D)]
         L.orc_sys_eval.argtypes = [P(hcg_tape), P(D), P(D), ctypes.c_int, P(D), P(D)]
@@ -101,7 +104,8 @@ class BatchResult:
         return [RETURN_CODES[c] for c in self.code]
 
 
-def track_batch(H, starts, opts: hcg_options | None = None, nthreads: int = 0, group_of_path=None) -> BatchResult:
+def track_batch(H, starts, opts: hcg_options | None = None, nthreads: int = 0, group_of_path=None,
+                log_capacity: int = 0) -> BatchResult:
     """Track every start with the oracle (threaded_solve restatement)."""
     opts = opts or default_options()
     starts = _c(starts)
@@ -127,11 +131,17 @@ def track_batch(H, starts, opts: hcg_options | None = None, nthreads: int = 0, g
     last = np.zeros((npaths, n), dtype=np.complex128)
     val = np.zeros((npaths, n), dtype=np.float64)
     nthreads = nthreads or os.cpu_count() or 1
-    rc = lib().orc_track_batch(ctypes.byref(H.desc()), ctypes.byref(b), ctypes.byref(opts), nthreads, res,
-                               _dp(sol), _dp(last), _dp(val))
+    log = np.zeros((max(log_capacity, 1), 16), dtype=np.float32)
+    cnt = ctypes.c_int(0)
+    rc = lib().orc_track_batch_log(ctypes.byref(H.desc()), ctypes.byref(b), ctypes.byref(opts), nthreads, res,
+                                   _dp(sol), _dp(last), _dp(val),
+                                   log.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if log_capacity > 0
+                                   else ctypes.POINTER(ctypes.c_float)(), log_capacity, ctypes.byref(cnt))
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
-    return BatchResult(list(res), sol, last, val)
+    out = BatchResult(list(res), sol, last, val)
+    out.log = log[: min(cnt.value, log_capacity)] if log_capacity > 0 else None
+    return out
 
 
 def track_plain(H, x1, t1=1.0, t0=0.0, opts: hcg_options | None = None, p=None, q=None):

@@ -949,6 +949,10 @@ typedef struct {
     double singular_start;
     long singular_steps;
     cplx *prediction, *prev_prediction;
+    /* per-step record log */
+    float* log_records;
+    atomic_int* log_count;
+    int log_capacity, path;
 } orc_tracker;
 
 static cplx* cvec(int n) { return calloc(n, sizeof(cplx)); }
@@ -1510,11 +1514,35 @@ static double refine_current_solution(orc_tracker* T, double min_tol, int nsteps
     return mu;
 }
 
+static void log_step(orc_tracker* T, cplx t, double ds, const newton_result* res, double le, double tau, double omega,
+                     double mu, int acc) {
+    int idx = atomic_fetch_add(T->log_count, 1);
+    if (idx >= T->log_capacity) return;
+    float* r = T->log_records + (size_t)idx * 16;
+    r[0] = (float)T->path;
+    r[1] = (float)(T->accepted + T->rejected - 1);
+    r[2] = (float)t.re;
+    r[3] = (float)hc_log(ds);
+    r[4] = (float)acc;
+    r[5] = (float)res->iters;
+    r[6] = (float)res->theta;
+    r[7] = (float)hc_log(le * powi(ds, 4));
+    r[8] = (float)(hc_log(T->cond_H_xd) / 2.302585092994046);
+    r[9] = (float)hc_log(omega);
+    r[10] = (float)hc_log(mu);
+    r[11] = (float)hc_log(tau);
+    r[12] = (float)hc_log(seg_ds(T));
+    r[13] = (float)(T->extended_prec + 2 * T->singular_endgame);
+    r[14] = (float)(hc_log(T->accuracy) / 2.302585092994046);
+    r[15] = (float)T->steps_eg;
+}
+
 /* step!(tracker), src/tracker.jl:851-926; returns !last_step_failed */
 static int tracker_step(orc_tracker* T) {
     const orc_options* o = T->opt;
     int n = T->n;
     cplx t = seg_t(T), dt = seg_dt(T), t1 = seg_t1(T);
+    const double ds0 = seg_ds(T), le0 = T->local_error, tau0 = T->tau, om0 = T->omega, mu0 = T->mu;
     predict(T, T->xh, T->x, t, dt);
     wnorm_update(T, T->xh);
     newton_result res = newton(T, T->xb, T->xh, t1, T->mu, T->omega, T->extended_prec, 0, T->accepted == 0);
@@ -1543,6 +1571,7 @@ static int tracker_step(orc_tracker* T) {
     }
     T->norm_dx0 = res.norm_dx0;
     update_stepsize(T, &res);
+    if (T->log_records) log_step(T, t, ds0, &res, le0, tau0, om0, mu0, res.code == NEWT_CONVERGED);
     check_terminated(T);
     return !(T->last_steps_failed > 0);
 }
@@ -1578,12 +1607,12 @@ static double finite_diff(double v, double s, double v2, double s2, double v1, d
 /* update!(val, pred, t), src/valuation.jl:82-124 */
 static void val_update(orc_tracker* T, double t) {
     int n = T->n;
-    double logt = log(t);
+    double logt = hc_log(t);
     double logt2 = T->logt2, logt1 = T->logt1;
     int diff = T->winding > 1 && !isnan(logt2);
     for (int i = 0; i < n; i++) {
         cplx xi = T->tx0[i], xdi = T->tx1[i], x2i = T->tx2[i], x3i = T->tx3[i];
-        double lxi = log(fast_abs(xi)), ldxi = log(fast_abs(xdi));
+        double lxi = hc_log(fast_abs(xi)), ldxi = hc_log(fast_abs(xdi));
         if (diff) {
             double vi = nu1(xi, xdi, t);
             double dvi = finite_diff(vi, logt, T->vx2[i], logt2, T->vx1[i], logt1);
@@ -1776,7 +1805,7 @@ static int check_at_infinity(orc_tracker* T) {
                 int at_zero = T->val_x[i] > 0;
                 double cond_growth = kappa / T->ai_conds[i];
                 double coord_growth = at_zero ? T->ai_abs_coords[i] / fast_abs(T->x[i]) : fast_abs(T->x[i]) / T->ai_abs_coords[i];
-                if (coord_growth > jclamp(pow(0.25, 4 * vi), 20, o->min_coord_growth) &&
+                if (coord_growth > jclamp(hc_pow(0.25, 4 * vi), 20, o->min_coord_growth) &&
                     (cond_growth > o->min_cond_growth || kappa > jmax(1e8, o->min_cond))) {
                     T->eg_code = at_zero ? EG_AT_ZERO : EG_AT_INFINITY;
                     return 1;
@@ -1972,6 +2001,9 @@ typedef struct {
     orc_path_result* res;
     double *sol, *last, *val;
     atomic_long next;
+    float* log_records;
+    int log_capacity;
+    atomic_int log_count;
 } job_t;
 
 static void set_path_params(orc_tracker* T, const job_t* J, long k) {
@@ -1992,6 +2024,10 @@ static void* worker(void* arg) {
     job_t* J = arg;
     orc_tracker T;
     tracker_create(&T, J->hd, J->o);
+    T.log_records = J->log_records;
+    T.log_capacity = J->log_capacity;
+    T.log_count = &J->log_count;
+    T.cond_H_xd = 1.0;
     int n = J->hd->n;
     cplx* x = cvec(n);
     for (;;) {
@@ -1999,6 +2035,7 @@ static void* worker(void* arg) {
         if (k >= J->b->npaths) break;
         for (int i = 0; i < n; i++) x[i] = C(J->b->starts[2 * (k * n + i)], J->b->starts[2 * (k * n + i) + 1]);
         set_path_params(&T, J, k);
+        T.path = (int)k;
         eg_track(&T, x, J->b->t1, &J->res[k], J->sol + 2 * k * n, J->last + 2 * k * n, J->val + k * n);
     }
     free(x);
@@ -2007,16 +2044,22 @@ static void* worker(void* arg) {
 }
 
 /* Batch solve over `nthreads` threads with a dynamic atomic queue (threaded_solve). */
-int orc_track_batch(const orc_homotopy_desc* hd, const orc_batch* b, const orc_options* o, int nthreads,
-                    orc_path_result* res, double* sol, double* last, double* val) {
+int orc_track_batch_log(const orc_homotopy_desc* hd, const orc_batch* b, const orc_options* o, int nthreads,
+                        orc_path_result* res, double* sol, double* last, double* val, float* log_records,
+                        int log_capacity, int* log_count) {
     if (check_desc(hd)) return -1;
     if (nthreads < 1) nthreads = 1;
-    job_t J = {hd, b, o, res, sol, last, val, 0};
+    job_t J = {hd, b, o, res, sol, last, val, 0, log_records, log_capacity, 0};
     pthread_t* th = malloc(sizeof(pthread_t) * nthreads);
     for (int i = 0; i < nthreads; i++) pthread_create(&th[i], NULL, worker, &J);
     for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
     free(th);
+    if (log_count) *log_count = atomic_load(&J.log_count);
     return 0;
+}
+int orc_track_batch(const orc_homotopy_desc* hd, const orc_batch* b, const orc_options* o, int nthreads,
+                    orc_path_result* res, double* sol, double* last, double* val) {
+    return orc_track_batch_log(hd, b, o, nthreads, res, sol, last, val, NULL, 0, NULL);
 }
 
 /* Plain Tracker API: track(tracker, x, t₁, t₀) -> TrackerResult (src/tracker.jl:1023-1026).
@@ -2159,5 +2202,14 @@ size_t orc_sizeof(int which) {
     case 4: return sizeof(orc_batch);
     case 5: return sizeof(orc_path_result);
     default: return 0;
+    }
+}
+/* deterministic transcendental kernels, for pinning: 0 log, 1 exp, 2 pow, 3 cbrt */
+double orc_math(int which, double x, double y) {
+    switch (which) {
+    case 0: return hc_log(x);
+    case 1: return hc_exp(x);
+    case 2: return hc_pow(x, y);
+    default: return hc_cbrt(x);
     }
 }

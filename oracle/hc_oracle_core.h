@@ -53,6 +53,64 @@ static inline cplx cdiv(cplx a, cplx b) {
 static inline cplx cinv(cplx a) { double d = cabs2(a); return C(a.re / d, -a.im / d); }
 static inline int cisnan(cplx a) { return isnan(a.re) || isnan(a.im); }
 
+
+/* Deterministic log / exp / pow / cbrt built from IEEE-exact operations only
+ * (+ - * / sqrt, frexp/ldexp), so that the CPU oracle and the GPU kernels produce
+ * identical bits; accuracy is about 1 ulp.  log: atanh series on m in [sqrt(1/2), sqrt(2));
+ * exp: k*ln2 range reduction + degree-16 Taylor; pow(x,y) = exp(y log x) for x > 0;
+ * cbrt: exp(log|x|/3) plus one Newton correction. */
+#define HC_LN2_HI 6.93147180369123816490e-01
+#define HC_LN2_LO 1.90821492927058770002e-10
+static inline double hc_log(double x) {
+    if (isnan(x) || x < 0) return NAN;
+    if (x == 0) return -INFINITY;
+    if (isinf(x)) return x;
+    int e;
+    double m = frexp(x, &e);               /* x = m 2^e, m in [0.5, 1) */
+    if (m < 0.70710678118654752440) { m = 2.0 * m; e -= 1; }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 1.0 / 25.0;
+    p = p * s2 + 1.0 / 23.0; p = p * s2 + 1.0 / 21.0; p = p * s2 + 1.0 / 19.0; p = p * s2 + 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0; p = p * s2 + 1.0 / 13.0; p = p * s2 + 1.0 / 11.0; p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0; p = p * s2 + 1.0 / 5.0; p = p * s2 + 1.0 / 3.0;
+    double lm = 2.0 * s + 2.0 * s * (s2 * p);
+    double de = (double)e;
+    return de * HC_LN2_HI + (de * HC_LN2_LO + lm);
+}
+static inline double hc_exp(double x) {
+    if (isnan(x)) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.2) return 0.0;
+    double kf = nearbyint(x * 1.44269504088896340736);
+    double r = (x - kf * HC_LN2_HI) - kf * HC_LN2_LO;
+    double p = 1.0 / 20922789888000.0;           /* 1/16! */
+    const double inv[16] = {1.0 / 1307674368000.0, 1.0 / 87178291200.0, 1.0 / 6227020800.0, 1.0 / 479001600.0,
+                            1.0 / 39916800.0, 1.0 / 3628800.0, 1.0 / 362880.0, 1.0 / 40320.0, 1.0 / 5040.0,
+                            1.0 / 720.0, 1.0 / 120.0, 1.0 / 24.0, 1.0 / 6.0, 0.5, 1.0, 1.0};
+    for (int i = 0; i < 16; i++) p = p * r + inv[i];
+    int k = (int)kf;
+    if (k > 1000) { p = ldexp(p, 1000); k -= 1000; }
+    if (k < -1000) { p = ldexp(p, -1000); k += 1000; }
+    return ldexp(p, k);
+}
+static inline double hc_pow(double x, double y) {
+    if (y == 0) return 1.0;
+    if (x == 1.0) return 1.0;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0) return y > 0 ? 0.0 : INFINITY;
+    if (isinf(x)) return y > 0 ? INFINITY : 0.0;
+    if (x < 0) return NAN;
+    return hc_exp(y * hc_log(x));
+}
+static inline double hc_cbrt(double x) {
+    if (x == 0 || isnan(x) || isinf(x)) return x;
+    double a = fabs(x);
+    double y = hc_exp(hc_log(a) / 3.0);
+    y = y - (y * y * y - a) / (3.0 * (y * y));
+    return x < 0 ? -y : y;
+}
+
 /* Julia's NaN-propagating min/max/clamp (Base.min, Base.max, Base.clamp). */
 static inline double jmin(double a, double b) { if (isnan(a) || isnan(b)) return NAN; return a < b ? a : b; }
 static inline double jmax(double a, double b) { if (isnan(a) || isnan(b)) return NAN; return a > b ? a : b; }
@@ -73,10 +131,10 @@ static inline double jeps(double x) {
 static inline double nthroot(double x, int N) {
     if (N == 4) return sqrt(sqrt(x));
     if (N == 2) return sqrt(x);
-    if (N == 3) return cbrt(x);
+    if (N == 3) return hc_cbrt(x);
     if (N == 1) return x;
     if (N == 0) return 1.0;
-    return pow(x, 1.0 / N);
+    return hc_pow(x, 1.0 / N);
 }
 /* integer power of a real by repeated squaring (used for s^(m-1), Δs^order) */
 static inline double powi(double x, int n) {

@@ -1,0 +1,100 @@
+"""GPU parity: libhcgpu.so (HIP, gfx950) against the CPU oracle on the same inputs.
+
+The bar: identical return codes and counts, identical singular / winding
+classification, endpoints equal within 1e-10 relative error for nonsingular
+solutions (the kernel follows the oracle's arithmetic contract, so trajectories
+normally agree bit for bit; the tolerance absorbs libm differences).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import systems
+from hcgpu import abi
+from hcgpu.configs import dense_system
+from hcgpu.homotopies import ParameterHomotopy, total_degree
+from hcgpu.solve import EndgameTracker, make_options
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def run_both(H, S, gpu_ctx, **opt):
+    opts = make_options(**opt)
+    T = EndgameTracker(H, ctx=gpu_ctx)
+    T.options = opts
+    g = T.track_batch(S)
+    o = oracle.track_batch(H, S, opts)
+    return g, o
+
+
+def assert_parity(g, o, endpoint_rtol=RTOL):
+    assert list(g.code) == list(o.code), "return codes differ"
+    assert list(g.singular) == list(o.singular)
+    assert list(g.winding_number) == list(o.winding_number)
+    ok = (o.code == 1) & (~o.singular)
+    if ok.any():
+        d = np.abs(g.solution[ok] - o.solution[ok]).max(axis=1)
+        scale = np.maximum(1.0, np.abs(o.solution[ok]).max(axis=1))
+        assert (d / scale).max() <= endpoint_rtol, f"endpoint mismatch {(d / scale).max():.3e}"
+
+
+def test_beyond_polyhedral(gpu_ctx):
+    H, S = total_degree(systems.beyond_polyhedral(), gamma=1.3j + 0.4)
+    g, o = run_both(H, S, gpu_ctx)
+    print("gpu", g.codes(), g.accepted_steps, "oracle", o.codes(), o.accepted_steps)
+    assert_parity(g, o)
+    # test/endgame_tracker_test.jl:2-46 on the GPU result
+    assert g.codes() == ["success", "success", "at_infinity", "at_infinity"]
+    assert g.accuracy[0] < 1e-12 and g.residual[0] < 1e-12
+    assert g.accepted_steps[0] < 20 and g.rejected_steps[0] == 0
+    assert g.cond[1] < 1e3
+    np.testing.assert_allclose(g.valuation[2], [-1, -1], rtol=1e-3)
+    assert (g.accepted_steps == o.accepted_steps).all()
+
+
+def test_parameter_homotopy_tracker_case(gpu_ctx):
+    F = systems.tracker_param_system()
+    H = ParameterHomotopy(F, [1, 0], [2, 4])
+    S = np.array([[1, 1]], dtype=complex)
+    g, o = run_both(H, S, gpu_ctx)
+    assert_parity(g, o)
+    np.testing.assert_allclose(g.solution[0], [np.sqrt(2), -np.sqrt(2)], rtol=1e-12)
+
+
+def test_dense_n4_config0(gpu_ctx):
+    F = dense_system(4, 3, seed=100)
+    H, S = total_degree(F, seed=101)
+    g, o = run_both(H, S, gpu_ctx)
+    assert_parity(g, o)
+    assert (g.code == 1).sum() == 81
+
+
+@pytest.mark.parametrize("name", ["cyclic7", "mohab", "hyperbolic", "winding4", "wilkinson12"])
+def test_known_answers_gpu(gpu_ctx, name):
+    from hcgpu.result import Result, path_results_from_batch
+    if name == "cyclic7":
+        H, S = total_degree(systems.cyclic(7), seed=4)
+    elif name == "mohab":
+        H, S = total_degree(systems.mohab(), gamma=systems.MOHAB_GAMMA)
+    elif name == "hyperbolic":
+        H, S = total_degree(systems.hyperbolic(), seed=5)
+    elif name == "winding4":
+        H, S = total_degree(systems.winding_family(4), seed=3)
+    else:
+        H, S = total_degree(systems.wilkinson(12), seed=2)
+    g, o = run_both(H, S, gpu_ctx)
+    assert_parity(g, o, endpoint_rtol=1e-8)
+    R = Result(path_results_from_batch(g, S))
+    if name == "cyclic7":
+        assert R.nsolutions() == 924
+    elif name == "mohab":
+        assert R.nnonsingular() == 693
+    elif name == "hyperbolic":
+        assert sum(1 for r in R if r.winding_number == 3) == 12
+        assert R.nresults() == 2 and R.nsingular() == 2
+    elif name == "winding4":
+        assert sum(1 for r in R if r.is_success()) == 5
+    else:
+        assert sorted(np.round(np.real([r.solution[0] for r in R])).astype(int)) == list(range(1, 13))
