@@ -54,7 +54,7 @@ struct Lane {
     int ipiv[NN];
     double rs[NN];
     int factorized, scaled;
-    cplx dx[NN], r[NN], tmp[NN];
+    cplx dx[NN], r[NN], tmp[NN], rdx[NN];  // rdx: the MatrixWorkspace's own correction buffer (δx)
     // Predictor
     int hermite, order, winding;
     double trust_region, local_error;
@@ -301,18 +301,18 @@ __device__ __noinline__ double mixed_refine(Lane& L, cplx* x, const cplx* b, con
         for (int i = 0; i < NN; i++) r[i] = cdd_add(r[i], cdd_mul_c(L.A[IDX(i, j)], xj));
     }
     for (int i = 0; i < NN; i++) L.tmp[i] = cdd_round(cdd_sub_c(r[i], b[i]));
-    ws_ldiv(L, L.dx, L.tmp);
-    for (int i = 0; i < NN; i++) x[i] = c_sub(x[i], L.dx[i]);
-    return norm_inf(L.dx, w) / norm_inf(x, w);
+    ws_ldiv(L, L.rdx, L.tmp);
+    for (int i = 0; i < NN; i++) x[i] = c_sub(x[i], L.rdx[i]);
+    return norm_inf(L.rdx, w) / norm_inf(x, w);
 }
 // fixed_precision_iterative_refinement!, src/linear_algebra.jl:553-567
 __device__ __noinline__ double fixed_refine(Lane& L, cplx* x, const cplx* b, const double* w) {
     for (int i = 0; i < NN; i++) L.tmp[i] = mkc(0.0, 0.0);
     for (int j = 0; j < NN; j++) { cplx xj = x[j]; for (int i = 0; i < NN; i++) L.tmp[i] = c_muladd(L.A[IDX(i, j)], xj, L.tmp[i]); }
     for (int i = 0; i < NN; i++) L.tmp[i] = c_sub(L.tmp[i], b[i]);
-    ws_ldiv(L, L.dx, L.tmp);
-    for (int i = 0; i < NN; i++) x[i] = c_sub(x[i], L.dx[i]);
-    return norm_inf(L.dx, w) / norm_inf(x, w);
+    ws_ldiv(L, L.rdx, L.tmp);
+    for (int i = 0; i < NN; i++) x[i] = c_sub(x[i], L.rdx[i]);
+    return norm_inf(L.rdx, w) / norm_inf(x, w);
 }
 // iterative_refinement!, src/linear_algebra.jl:864-885
 __device__ __noinline__ double iterative_refinement(Lane& L, cplx* x, const cplx* b, const double* w, double tol, int max_iters) {

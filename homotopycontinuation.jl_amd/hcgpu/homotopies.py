@@ -22,7 +22,11 @@ from .tape import Tape, tapes
 class _SysTapes:
     def __init__(self, system: System, fixed_params: Optional[Sequence[complex]] = None):
         self.system = system
-        self.eval_jac, self.eval = tapes(system)
+        if hasattr(system, "exprs"):
+            from .expr import expr_tapes
+            self.eval_jac, self.eval = expr_tapes(system)
+        else:
+            self.eval_jac, self.eval = tapes(system)
         self.fixed = None
         if fixed_params is not None:
             self.fixed = np.ascontiguousarray(np.asarray(fixed_params, dtype=np.complex128).view(np.float64))

@@ -230,3 +230,24 @@ def norm(x, w=None):
     null = ctypes.POINTER(ctypes.c_double)()
     ww = np.ascontiguousarray(w, dtype=np.float64) if w is not None else None
     return lib().orc_norm(x.shape[0], _dp(x), _dp(ww) if ww is not None else null)
+
+
+def track_plain_valuation(H, x1, t1, t0, opts: hcg_options | None = None):
+    opts = opts or default_options()
+    L = lib()
+    L.orc_track_plain_valuation.argtypes = [ctypes.POINTER(hcg_homotopy_desc), ctypes.POINTER(hcg_options),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    x1 = _c(x1)
+    n = x1.shape[0]
+    vx = np.zeros(n)
+    vt = np.zeros(n)
+    code = L.orc_track_plain_valuation(ctypes.byref(H.desc()), ctypes.byref(opts), _dp(x1), t1, t0, _dp(vx), _dp(vt))
+    return TRACKER_CODES[code], vx, vt
+
+
+def math_fn(which, x, y=0.0):
+    L = lib()
+    L.orc_math.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+    L.orc_math.restype = ctypes.c_double
+    return L.orc_math(which, x, y)

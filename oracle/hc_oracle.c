@@ -2213,3 +2213,23 @@ double orc_math(int which, double x, double y) {
     default: return hc_cbrt(x);
     }
 }
+/* Track x₁ from t₁ to t₀ with the plain tracker, then update a fresh Valuation at the final t
+ * (the reference's valuation tests, test/valuation_test.jl). */
+int orc_track_plain_valuation(const orc_homotopy_desc* hd, const orc_options* o, const double* x1, double t1, double t0,
+                              double* val_x, double* val_tx) {
+    if (check_desc(hd)) return -1;
+    orc_tracker T;
+    tracker_create(&T, hd, o);
+    int n = hd->n;
+    cplx* x = cvec(n);
+    for (int i = 0; i < n; i++) x[i] = C(x1[2 * i], x1[2 * i + 1]);
+    tracker_init(&T, x, C(t1, 0), C(t0, 0), NAN, NAN, 0);
+    while (T.code == TC_TRACKING) tracker_step(&T);
+    val_init(&T);
+    val_update(&T, seg_t(&T).re);
+    for (int i = 0; i < n; i++) { val_x[i] = T.val_x[i]; val_tx[i] = T.val_tx[i]; }
+    int code = T.code;
+    free(x);
+    tracker_destroy(&T);
+    return code;
+}
