@@ -526,6 +526,21 @@ bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, st
     } else {
         gen_system(os, h->target, "sysF", n, PM_DYNAMIC, false, fl);
     }
+    // static FP64 flop counts of one homotopy-level call (system functions + combination)
+    long comb_eval = 0, comb_ej = 0, comb_dd = 0, comb_t1 = 0, comb_t23 = 0;
+    if (h->kind == HCG_STRAIGHT_LINE) {
+        comb_eval = 6 + 14L * n;
+        comb_ej = 6 + 14L * (n + (long)n * n);
+        comb_dd = 6 + 94L * n;
+        comb_t1 = 8L * n;
+        comb_t23 = 24L * n;
+    }
+    os << "#define HCG_FL_EVAL " << fl->eval + comb_eval << "\n";
+    os << "#define HCG_FL_EJ " << fl->eval_jac + comb_ej << "\n";
+    os << "#define HCG_FL_DD " << fl->eval_dd + comb_dd << "\n";
+    os << "#define HCG_FL_T1 " << (h->kind == HCG_STRAIGHT_LINE ? fl->eval + comb_t1 : fl->taylor[1]) << "\n";
+    os << "#define HCG_FL_T2 " << fl->taylor[2] + comb_t23 << "\n";
+    os << "#define HCG_FL_T3 " << fl->taylor[3] + comb_t23 << "\n";
     os << "#include \"hcg_tracker.h\"\n";
     *src = os.str();
     return true;

@@ -6,6 +6,8 @@
 typedef struct hcg_kargs {
     int32_t npaths;
     int32_t log_capacity;
+    int32_t ngroups;
+    int32_t pad_;
     const double* starts;          // npaths * n complex
     double t1, t0;
     const double* start_params;    // nparams complex (PARAMETER)
@@ -17,7 +19,7 @@ typedef struct hcg_kargs {
     double* solutions;
     double* last_points;
     double* valuations;
-    unsigned long long* counters;  // [0] tracker steps, [1] Newton iterations, [2] learned proposals, [3] accepted
+    unsigned long long* counters;  // [0] tracker steps, [1] Newton iterations, [2] counted FP64 flops, [3] learned proposals accepted
     float* log_records;            // step log (HCG_STEP_RECORD_FLOATS per record) or NULL
     int32_t* log_count;
     const float* mlp;              // learned proposer weights (FP32 master copy) or NULL

@@ -32,6 +32,12 @@ enum { NEWT_CONVERGED = 0, NEWT_TERMINATED, NEWT_MAX_ITERS, NEWT_SINGULARITY };
 
 #define IDX(i, j) ((j) * NN + (i))
 
+// FP64 flop counts of the dense linear algebra (complex: mul 6, muladd 8, div 11)
+#define FL_LU (11.0 * (NN * (NN - 1) / 2) + 8.0 * ((NN - 1) * NN * (2 * NN - 1) / 6))
+#define FL_SOLVE (8.0 * NN * (NN - 1) + 11.0 * NN)
+#define FL_MIXED (80.0 * NN * NN + 26.0 * NN + FL_SOLVE)
+#define FL_FIXED (8.0 * NN * NN + 4.0 * NN + FL_SOLVE)
+
 struct Lane {
     const hcg_options* o;
     // ParameterHomotopy data
@@ -86,6 +92,7 @@ struct Lane {
     // learned proposer bookkeeping
     int learned_proposed, learned_accepted, last_learned;
     double last_theta, cond_H_xd;
+    double flops;  // counted FP64 flops of this path
     // step log
     float* log_records;
     int* log_count;
@@ -111,6 +118,7 @@ __device__ void H_tp(Lane& L, cplx t) {
 #endif
 
 __device__ __noinline__ void H_eval(Lane& L, const cplx* x, cplx t, cplx* u) {
+    L.flops += HCG_FL_EVAL;
 #if HCG_KIND == 0
     cplx u2[NN];
     sysG_eval(x, u);
@@ -124,6 +132,7 @@ __device__ __noinline__ void H_eval(Lane& L, const cplx* x, cplx t, cplx* u) {
 }
 
 __device__ __noinline__ void H_eval_dd(Lane& L, const cplx* x, cplx t, cplx* u) {
+    L.flops += HCG_FL_DD;
     cdd xd[NN], u1[NN];
     for (int i = 0; i < NN; i++) xd[i] = cdd_from(x[i]);
 #if HCG_KIND == 0
@@ -141,6 +150,7 @@ __device__ __noinline__ void H_eval_dd(Lane& L, const cplx* x, cplx t, cplx* u) 
 
 // evaluate_and_jacobian!; U column-major
 __device__ __noinline__ void H_eval_jac(Lane& L, const cplx* x, cplx t, cplx* u, cplx* U) {
+    L.flops += HCG_FL_EJ;
 #if HCG_KIND == 0
     cplx u2[NN];
     cplx U2[NN * NN];
@@ -157,6 +167,7 @@ __device__ __noinline__ void H_eval_jac(Lane& L, const cplx* x, cplx t, cplx* u,
 
 // taylor!(u, Val(K), H, tx, t) with tx = (tx0, ..., tx_{K-1})
 __device__ __noinline__ void H_taylor(Lane& L, int K, cplx t, cplx* u) {
+    L.flops += K == 1 ? HCG_FL_T1 : (K == 2 ? HCG_FL_T2 : HCG_FL_T3);
 #if HCG_KIND == 0
     cplx g = mkc(HCG_GAMMA_RE, HCG_GAMMA_IM);
     if (K == 1) {
@@ -192,6 +203,7 @@ __device__ __forceinline__ void ws_updated(Lane& L) {
 // lu!, src/linear_algebra.jl:130-184
 __device__ __noinline__ void lu_factor(Lane& L) {
     cplx* A = L.LU;
+    L.flops += FL_LU;
     for (int k = 0; k < NN; k++) {
         int kp = k;
         double amax = c_abs2(A[IDX(k, k)]);
@@ -214,8 +226,9 @@ __device__ __noinline__ void lu_factor(Lane& L) {
     L.factorized = 1;
 }
 // lu_ldiv!, src/linear_algebra.jl:268-316 (in place)
-__device__ __noinline__ void lu_solve(const Lane& L, cplx* x) {
+__device__ __noinline__ void lu_solve(Lane& L, cplx* x) {
     const cplx* A = L.LU;
+    L.flops += FL_SOLVE;
     for (int i = 0; i < NN; i++) { int p = L.ipiv[i]; if (i != p) { cplx t = x[i]; x[i] = x[p]; x[p] = t; } }
     for (int j = 0; j < NN; j++) {
         cplx xj = x[j];
@@ -228,8 +241,9 @@ __device__ __noinline__ void lu_solve(const Lane& L, cplx* x) {
     }
 }
 // lu_ldiv_adj!, src/linear_algebra.jl:318-354
-__device__ __noinline__ void lu_solve_adj(const Lane& L, cplx* x) {
+__device__ __noinline__ void lu_solve_adj(Lane& L, cplx* x) {
     const cplx* A = L.LU;
+    L.flops += FL_SOLVE;
     for (int j = 0; j < NN; j++) {
         cplx z = x[j];
         for (int i = 0; i < j; i++) z = c_submul(c_conj(A[IDX(i, j)]), x[i], z);
@@ -293,6 +307,7 @@ __device__ __noinline__ double dist_inf(const cplx* x, const cplx* y, const doub
 }
 // mixed_precision_iterative_refinement!, src/linear_algebra.jl:528-544 (residual in double-double)
 __device__ __noinline__ double mixed_refine(Lane& L, cplx* x, const cplx* b, const double* w) {
+    L.flops += FL_MIXED - FL_SOLVE;
     cdd r[NN];
     cdd z = mkcdd(mkdd(0.0, 0.0), mkdd(0.0, 0.0));
     for (int i = 0; i < NN; i++) r[i] = z;
@@ -307,6 +322,7 @@ __device__ __noinline__ double mixed_refine(Lane& L, cplx* x, const cplx* b, con
 }
 // fixed_precision_iterative_refinement!, src/linear_algebra.jl:553-567
 __device__ __noinline__ double fixed_refine(Lane& L, cplx* x, const cplx* b, const double* w) {
+    L.flops += FL_FIXED - FL_SOLVE;
     for (int i = 0; i < NN; i++) L.tmp[i] = mkc(0.0, 0.0);
     for (int j = 0; j < NN; j++) { cplx xj = x[j]; for (int i = 0; i < NN; i++) L.tmp[i] = c_muladd(L.A[IDX(i, j)], xj, L.tmp[i]); }
     for (int i = 0; i < NN; i++) L.tmp[i] = c_sub(L.tmp[i], b[i]);
@@ -1355,9 +1371,26 @@ __device__ __noinline__ void write_result(Lane& L, const hcg_kargs& a, int k) {
     R.last_t = L.last_t;
     a.results[k] = R;
     for (int i = 0; i < NN; i++) {
-        a.solutions[2 * (k * NN + i)] = s[i].re; a.solutions[2 * (k * NN + i) + 1] = s[i].im;
-        a.last_points[2 * (k * NN + i)] = L.last_point[i].re; a.last_points[2 * (k * NN + i) + 1] = L.last_point[i].im;
-        a.valuations[k * NN + i] = L.val_x[i];
+        size_t o = (size_t)k * NN + i;
+        a.solutions[2 * o] = s[i].re; a.solutions[2 * o + 1] = s[i].im;
+        a.last_points[2 * o] = L.last_point[i].re; a.last_points[2 * o + 1] = L.last_point[i].im;
+        a.valuations[o] = L.val_x[i];
+    }
+}
+
+// record of a path whose inputs were rejected on the device (out-of-range parameter group)
+__device__ __noinline__ void write_result_invalid(Lane& L, const hcg_kargs& a, int k) {
+    hcg_path_result R;
+    R.code = EG_UNKNOWN;
+    R.singular = R.winding_number = R.extended_precision = R.extended_precision_used = 0;
+    R.accepted_steps = R.rejected_steps = R.steps_eg = R.valuation_valid = R.newton_iters = 0;
+    R.learned_proposed = R.learned_accepted = 0;
+    R.t = a.t1; R.accuracy = R.residual = R.cond = HCG_NAN; R.omega = R.mu = R.last_t = HCG_NAN;
+    a.results[k] = R;
+    for (int i = 0; i < NN; i++) {
+        a.solutions[2 * ((size_t)k * NN + i)] = HCG_NAN; a.solutions[2 * ((size_t)k * NN + i) + 1] = HCG_NAN;
+        a.last_points[2 * ((size_t)k * NN + i)] = HCG_NAN; a.last_points[2 * ((size_t)k * NN + i) + 1] = HCG_NAN;
+        a.valuations[(size_t)k * NN + i] = HCG_NAN;
     }
 }
 
@@ -1372,19 +1405,27 @@ extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
     int k = -1;
     int guard = 0;  // loop iterations of the current path (each performs >= 1 tracker step)
     long long my_steps = 0, my_newton = 0;
+    double my_flops = 0.0;
     while (true) {
         if (phase == 0) {
             k = atomicAdd(a.next_path, 1);
             if (k >= a.npaths) break;
-            for (int i = 0; i < NN; i++) L.x[i] = mkc(a.starts[2 * (k * NN + i)], a.starts[2 * (k * NN + i) + 1]);
+            for (int i = 0; i < NN; i++) L.x[i] = mkc(a.starts[2 * ((size_t)k * NN + i)], a.starts[2 * ((size_t)k * NN + i) + 1]);
+            L.path = k;
+            L.flops = 0.0;
 #if HCG_KIND == 1
             {
                 int g = a.group_of_path ? a.group_of_path[k] : 0;
+                if (g < 0 || g >= a.ngroups) {  // device-resident input: reject out-of-range groups
+                    L.eg_code = EG_UNKNOWN;
+                    L.accepted = L.rejected = L.steps_eg = L.newton_iters = 0;
+                    write_result_invalid(L, a, k);
+                    continue;
+                }
                 L.p = a.start_params;
                 L.q = a.target_params + 2 * HCG_NPAR * g;
             }
 #endif
-            L.path = k;
             eg_init(L, a.t1);
             guard = 0;
             phase = 1;
@@ -1396,10 +1437,12 @@ extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
             write_result(L, a, k);
             my_steps += L.accepted + L.rejected;
             my_newton += L.newton_iters;
+            my_flops += L.flops;
             phase = 0;
         }
     }
     atomicAdd(&a.counters[0], (unsigned long long)my_steps);
     atomicAdd(&a.counters[1], (unsigned long long)my_newton);
+    atomicAdd(&a.counters[2], (unsigned long long)my_flops);
 }
 

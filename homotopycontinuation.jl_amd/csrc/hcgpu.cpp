@@ -57,6 +57,9 @@ struct hcg_ctx {
     unsigned long long last_counters[4] = {0, 0, 0, 0};
     float* d_mlp = nullptr;
     bool mlp_loaded = false;
+    int32_t* d_next = nullptr;
+    unsigned long long* d_counters = nullptr;
+    hcg_options* d_opts = nullptr;
     hcg::MlpWork mlp_work;
 };
 
@@ -96,6 +99,9 @@ extern "C" void hcg_ctx_destroy(hcg_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->d_mlp) hipFree(c->d_mlp);
+    if (c->d_next) hipFree(c->d_next);
+    if (c->d_counters) hipFree(c->d_counters);
+    if (c->d_opts) hipFree(c->d_opts);
     hcg::mlp_work_free(&c->mlp_work);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -297,53 +303,120 @@ struct DevBuf {
 
 #include "device/hcg_kargs.h"
 
-extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
-                               hcg_path_result* results, double* solutions, double* last_points, double* valuations,
-                               hcg_step_log* log) {
-    if (!ctx || !prog || !b || !opts || !results || !solutions || !last_points || !valuations)
-        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+static int validate_batch_args(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                               bool host) {
+    if (!ctx || !prog || !b || !opts) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
-    const int n = prog->n;
     const long np = b->npaths;
     if (np < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "negative path count");
-    if (np == 0) return HCG_OK;
-    if (np > (1L << 30) / n) return fail(HCG_ERR_INVALID_ARGUMENT, "batch too large");
-    if (!b->starts) return fail(HCG_ERR_INVALID_ARGUMENT, "starts is NULL");
-    if (prog->kind == HCG_PARAMETER) {
+    if (np > (1L << 30) / prog->n) return fail(HCG_ERR_INVALID_ARGUMENT, "batch too large");
+    if (np > 0 && !b->starts) return fail(HCG_ERR_INVALID_ARGUMENT, "starts is NULL");
+    if (prog->kind == HCG_PARAMETER && np > 0) {
         if (!b->start_params || !b->target_params || b->ngroups < 1)
             return fail(HCG_ERR_INVALID_ARGUMENT, "parameter homotopy needs start/target parameters and ngroups >= 1");
-        if (b->group_of_path)
+        if (host && b->group_of_path)
             for (long k = 0; k < np; k++)
                 if (b->group_of_path[k] < 0 || b->group_of_path[k] >= b->ngroups)
                     return fail(HCG_ERR_INVALID_ARGUMENT, "group_of_path[%ld] = %d out of range", k, b->group_of_path[k]);
     }
-    if (opts->max_winding_number < 1 || opts->min_newton_iters < 0 || opts->min_newton_iters > 20)
+    if (opts->max_winding_number < 1 || opts->min_newton_iters < 0 || opts->min_newton_iters > 20 || opts->max_steps < 0)
         return fail(HCG_ERR_INVALID_ARGUMENT, "invalid options");
     if (opts->learned_step && !ctx->mlp_loaded) return fail(HCG_ERR_INVALID_ARGUMENT, "learned_step requested but no MLP loaded");
-    HIPCHK(hipSetDevice(ctx->device));
+    return HCG_OK;
+}
 
+// Persistent per-context device scalars: work-queue head, counters, options.
+static int ctx_scalars(hcg_ctx* ctx) {
+    if (ctx->d_next) return HCG_OK;
+    HIPCHK(hipMalloc(&ctx->d_next, sizeof(int32_t)));
+    HIPCHK(hipMalloc(&ctx->d_counters, sizeof(unsigned long long) * 4));
+    HIPCHK(hipMalloc(&ctx->d_opts, sizeof(hcg_options)));
+    return HCG_OK;
+}
+
+// Launch the persistent tracker kernel over device-resident inputs/outputs.
+static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                        hcg_path_result* d_res, double* d_sol, double* d_last, double* d_val, float* d_log, int logcap,
+                        int32_t* d_logc) {
+    int rc = ctx_scalars(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(int32_t), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * 4, ctx->stream));
+    hcg_kargs ka;
+    std::memset(&ka, 0, sizeof ka);
+    ka.npaths = b->npaths;
+    ka.log_capacity = logcap;
+    ka.ngroups = b->ngroups;
+    ka.starts = b->starts;
+    ka.t1 = b->t1;
+    ka.t0 = b->t0;
+    ka.start_params = b->start_params;
+    ka.target_params = b->target_params;
+    ka.group_of_path = b->group_of_path;
+    ka.next_path = ctx->d_next;
+    ka.opts = ctx->d_opts;
+    ka.results = d_res;
+    ka.solutions = d_sol;
+    ka.last_points = d_last;
+    ka.valuations = d_val;
+    ka.counters = ctx->d_counters;
+    ka.log_records = d_log;
+    ka.log_count = d_logc;
+    ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
+    const int block = 64;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
+    long waves_needed = ((long)b->npaths + block - 1) / block;
+    long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
+    if (grid < 1) grid = 1;
+    void* args[] = {&ka};
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->last_counters, ctx->d_counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    return HCG_OK;
+}
+
+static int finish_launch(hcg_ctx* ctx) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    return HCG_OK;
+}
+
+extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                               hcg_path_result* results, double* solutions, double* last_points, double* valuations,
+                               hcg_step_log* log) {
+    int rc = validate_batch_args(ctx, prog, b, opts, true);
+    if (rc) return rc;
+    if (!results || !solutions || !last_points || !valuations) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL output");
+    const int n = prog->n;
+    const long np = b->npaths;
+    if (np == 0) return HCG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
     const size_t sz_starts = sizeof(double) * 2 * n * np;
-    DevBuf d_starts, d_sp, d_tp, d_grp, d_next, d_opts, d_res, d_sol, d_last, d_val, d_cnt, d_log, d_logc;
+    DevBuf d_starts, d_sp, d_tp, d_grp, d_res, d_sol, d_last, d_val, d_log, d_logc;
     HIPCHK(hipMalloc(&d_starts.p, sz_starts));
-    HIPCHK(hipMalloc(&d_next.p, sizeof(int32_t)));
-    HIPCHK(hipMalloc(&d_opts.p, sizeof(hcg_options)));
     HIPCHK(hipMalloc(&d_res.p, sizeof(hcg_path_result) * np));
     HIPCHK(hipMalloc(&d_sol.p, sz_starts));
     HIPCHK(hipMalloc(&d_last.p, sz_starts));
     HIPCHK(hipMalloc(&d_val.p, sizeof(double) * n * np));
-    HIPCHK(hipMalloc(&d_cnt.p, sizeof(unsigned long long) * 4));
     HIPCHK(hipMemcpyAsync(d_starts.p, b->starts, sz_starts, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(d_opts.p, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemsetAsync(d_next.p, 0, sizeof(int32_t), ctx->stream));
-    HIPCHK(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long) * 4, ctx->stream));
+    hcg_batch db = *b;
+    db.starts = (const double*)d_starts.p;
     if (prog->kind == HCG_PARAMETER) {
         HIPCHK(hipMalloc(&d_sp.p, sizeof(double) * 2 * prog->npar));
         HIPCHK(hipMalloc(&d_tp.p, sizeof(double) * 2 * prog->npar * b->ngroups));
         HIPCHK(hipMemcpyAsync(d_sp.p, b->start_params, sizeof(double) * 2 * prog->npar, hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(hipMemcpyAsync(d_tp.p, b->target_params, sizeof(double) * 2 * prog->npar * b->ngroups, hipMemcpyHostToDevice, ctx->stream));
+        db.start_params = (const double*)d_sp.p;
+        db.target_params = (const double*)d_tp.p;
         if (b->group_of_path) {
             HIPCHK(hipMalloc(&d_grp.p, sizeof(int32_t) * np));
             HIPCHK(hipMemcpyAsync(d_grp.p, b->group_of_path, sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx->stream));
+            db.group_of_path = (const int32_t*)d_grp.p;
         }
     }
     int logcap = 0;
@@ -354,56 +427,39 @@ extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_
         HIPCHK(hipMalloc(&d_logc.p, sizeof(int32_t)));
         HIPCHK(hipMemsetAsync(d_logc.p, 0, sizeof(int32_t), ctx->stream));
     }
-
-    hcg_kargs ka;
-    std::memset(&ka, 0, sizeof ka);
-    ka.npaths = (int32_t)np;
-    ka.log_capacity = logcap;
-    ka.starts = (const double*)d_starts.p;
-    ka.t1 = b->t1;
-    ka.t0 = b->t0;
-    ka.start_params = (const double*)d_sp.p;
-    ka.target_params = (const double*)d_tp.p;
-    ka.group_of_path = (const int32_t*)d_grp.p;
-    ka.next_path = (int32_t*)d_next.p;
-    ka.opts = (const hcg_options*)d_opts.p;
-    ka.results = (hcg_path_result*)d_res.p;
-    ka.solutions = (double*)d_sol.p;
-    ka.last_points = (double*)d_last.p;
-    ka.valuations = (double*)d_val.p;
-    ka.counters = (unsigned long long*)d_cnt.p;
-    ka.log_records = (float*)d_log.p;
-    ka.log_count = (int32_t*)d_logc.p;
-    ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
-
-    const int block = 64;
-    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
-    long waves_needed = (np + block - 1) / block;
-    long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
-    if (grid < 1) grid = 1;
-    void* args[] = {&ka};
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    rc = launch_track(ctx, prog, &db, opts, (hcg_path_result*)d_res.p, (double*)d_sol.p, (double*)d_last.p,
+                      (double*)d_val.p, (float*)d_log.p, logcap, (int32_t*)d_logc.p);
+    if (rc) return rc;
     HIPCHK(hipMemcpyAsync(results, d_res.p, sizeof(hcg_path_result) * np, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(solutions, d_sol.p, sz_starts, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(last_points, d_last.p, sz_starts, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(valuations, d_val.p, sizeof(double) * n * np, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->last_counters, d_cnt.p, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (logcap >= 0 && d_logc.p) {
+    rc = finish_launch(ctx);
+    if (rc) return rc;
+    if (d_logc.p) {
         int32_t cnt = 0;
-        HIPCHK(hipMemcpyAsync(&cnt, d_logc.p, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(&cnt, d_logc.p, sizeof(int32_t), hipMemcpyDeviceToHost));
         *log->count = cnt;
         int ncopy = std::min(cnt, logcap);
         if (ncopy > 0)
             HIPCHK(hipMemcpy(log->records, d_log.p, sizeof(float) * HCG_STEP_RECORD_FLOATS * ncopy, hipMemcpyDeviceToHost));
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    ctx->last_ms = ms;
     return HCG_OK;
+}
+
+extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                      const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
+                                      double* last_points_dev, double* valuations_dev) {
+    int rc = validate_batch_args(ctx, prog, batch_dev, opts, false);
+    if (rc) return rc;
+    if (!results_dev || !solutions_dev || !last_points_dev || !valuations_dev)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL output");
+    if (batch_dev->npaths == 0) return HCG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    rc = launch_track(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, nullptr,
+                      0, nullptr);
+    if (rc) return rc;
+    return finish_launch(ctx);
 }
 
 extern "C" int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps, int64_t* total_newton_iters) {

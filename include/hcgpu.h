@@ -20,6 +20,9 @@
  */
 #ifndef HCGPU_H
 #define HCGPU_H
+#ifndef __HIPCC_RTC__
+#include <stddef.h>
+#endif
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -185,10 +188,26 @@ int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batc
                     const hcg_options* opts, hcg_path_result* results, double* solutions,
                     double* last_points, double* valuations, hcg_step_log* log);
 
+/* Same as hcg_track_batch, but every array reachable from `batch_dev` and every output
+ * pointer is device memory (HBM) of the context's device: nothing is copied, for
+ * callers that keep start solutions and results resident.  Out-of-range
+ * group_of_path entries are rejected per path on the device (code terminated_unknown). */
+int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                           const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
+                           double* last_points_dev, double* valuations_dev);
+
 /* Time of the last hcg_track_batch's tracker kernel (ms, HIP events on its stream),
  * and the device-side counters of that launch. */
 int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps,
                           int64_t* total_newton_iters);
+/* Device counters of the last launch: [0] tracker steps, [1] Newton iterations,
+ * [2] counted FP64 flops, [3] accepted learned proposals. */
+int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4);
+/* Generated HIP source and static flop counts (eval, eval_jac, eval_dd, taylor1..3) of a
+ * homotopy without compiling it; no device needed. */
+int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6);
+/* Generate and hipRTC-compile (filling the code-object cache); no device needed. */
+int hcg_compile_only(const hcg_homotopy_desc* h, size_t* code_size);
 
 /* Learned step-size proposer: 8 features -> 64 -> 64 (GELU) -> 2 outputs
  * (log step, at-infinity logit).  Weights are BF16 with FP32 accumulation. */
