@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run one plain device-to-device copy of this many MiB (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counter readings) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: paths are independent and this build has no multi-GPU path "
@@ -50,6 +53,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "read_bytes": nbytes, "written_bytes": nbytes}))
+        return
 
     from hcgpu import abi
     from hcgpu.abi import check, hcg_batch, hcg_path_result, lib

@@ -34,6 +34,19 @@ HD cplx c_mulsub(cplx a, cplx b, cplx c) {  // a*b - c
 }
 HD double c_abs2(cplx a) { return fma(a.re, a.re, a.im * a.im); }
 HD double c_fabs(cplx a) { return sqrt(c_abs2(a)); }
+// Base.abs(::Complex) = hypot(re, im), overflow-free; power-of-two scaling is exact, so the
+// result is the same on every device (IEEE-exact operations only)
+HD double c_hypot(cplx a) {
+    double x = fabs(a.re), y = fabs(a.im);
+    if (isinf(x) || isinf(y)) return HCG_INF;
+    if (isnan(x) || isnan(y)) return HCG_NAN;
+    double m = x > y ? x : y;
+    if (m == 0) return 0.0;
+    int e;
+    frexp(m, &e);
+    double xs = ldexp(x, -e), ys = ldexp(y, -e);
+    return ldexp(sqrt(fma(xs, xs, ys * ys)), e);
+}
 HD cplx c_scale(double s, cplx a) { return mkc(s * a.re, s * a.im); }
 HD cplx c_divr(cplx a, double s) { return mkc(a.re / s, a.im / s); }
 HD cplx c_div(cplx a, cplx b) {

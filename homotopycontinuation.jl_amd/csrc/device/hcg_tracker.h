@@ -283,8 +283,8 @@ __device__ __noinline__ double norm_inf(const cplx* x, const double* w) {
     for (int i = 1; i < NN; i++) dmax = max_fast(dmax, w ? c_abs2(c_divr(x[i], w[i])) : c_abs2(x[i]));
     double d = sqrt(dmax);
     if (isinf(d)) {
-        dmax = w ? c_fabs(c_divr(x[0], w[0])) : c_fabs(x[0]);
-        for (int i = 1; i < NN; i++) dmax = jmax(dmax, w ? c_fabs(c_divr(x[i], w[i])) : c_fabs(x[i]));
+        dmax = w ? c_hypot(c_divr(x[0], w[0])) : c_hypot(x[0]);
+        for (int i = 1; i < NN; i++) dmax = jmax(dmax, w ? c_hypot(c_divr(x[i], w[i])) : c_hypot(x[i]));
         return dmax;
     }
     return d;
@@ -295,10 +295,10 @@ __device__ __noinline__ double dist_inf(const cplx* x, const cplx* y, const doub
     double d = sqrt(dmax);
     if (isinf(d)) {
         if (w) {
-            dmax = c_fabs(c_divr(c_sub(x[0], y[0]), w[0]));
-            for (int i = 1; i < NN; i++) dmax = jmax(dmax, c_fabs(c_divr(c_sub(x[i], y[i]), w[i])));
+            dmax = c_hypot(c_divr(c_sub(x[0], y[0]), w[0]));
+            for (int i = 1; i < NN; i++) dmax = jmax(dmax, c_hypot(c_divr(c_sub(x[i], y[i]), w[i])));
         } else {  // the reference's unweighted fallback mixes abs and abs2 (src/norm.jl:160-163)
-            dmax = c_fabs(c_sub(x[0], y[0]));
+            dmax = c_hypot(c_sub(x[0], y[0]));
             for (int i = 1; i < NN; i++) dmax = jmax(dmax, c_abs2(c_sub(x[i], y[i])));
         }
         return dmax;
@@ -412,7 +412,7 @@ __device__ __noinline__ double inf_norm_ws(const Lane& L, const double* dl, cons
 // LA.cond(WS, d_l, d_r), src/linear_algebra.jl:745-774
 __device__ __noinline__ double ws_cond(Lane& L, const double* dl, const double* dr) {
     if (NN == 1) {
-        double a = sqrt(L.A[0].re * L.A[0].re + L.A[0].im * L.A[0].im);
+        double a = c_hypot(L.A[0]);
         if (!dl && !dr) return 1.0 / a;
         if (!dl) return 1.0 / (a * dr[0]);
         if (!dr) return 1.0 / (dl[0] * a);
@@ -463,8 +463,8 @@ __device__ __noinline__ int rank_rtol(Lane& L, double rtol) {
 __device__ __forceinline__ void seg_init(Lane& L, cplx start, cplx target) {
     L.seg_start = start; L.seg_target = target;
     cplx d = c_sub(target, start);
-    L.seg_absD = hypot(d.re, d.im);
-    L.seg_forward = hypot(start.re, start.im) < hypot(target.re, target.im);
+    L.seg_absD = c_hypot(d);
+    L.seg_forward = c_hypot(start) < c_hypot(target);
     L.seg_s = L.seg_s1 = L.seg_forward ? 0.0 : L.seg_absD;
 }
 __device__ __forceinline__ bool seg_is_done(const Lane& L) { return L.seg_forward ? L.seg_s == L.seg_absD : L.seg_s == 0.0; }

@@ -152,6 +152,8 @@ def track_plain(H, x1, t1=1.0, t0=0.0, opts: hcg_options | None = None, p=None, 
     xo = np.zeros(n, dtype=np.complex128)
     io = (ctypes.c_int32 * 5)()
     do = np.zeros(6)
+    if p is None and hasattr(H, "p"):
+        p, q = H.p, H.q[0]
     pp = _c(p) if p is not None else np.zeros(1, dtype=np.complex128)
     qq = _c(q) if q is not None else np.zeros(1, dtype=np.complex128)
     rc = lib().orc_track_plain(ctypes.byref(H.desc()), ctypes.byref(opts), _dp(pp), _dp(qq), _dp(x1), t1, t0,
@@ -251,3 +253,21 @@ def math_fn(which, x, y=0.0):
     L.orc_math.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
     L.orc_math.restype = ctypes.c_double
     return L.orc_math(which, x, y)
+
+
+def segment_run(start, target, ops):
+    """Run a SegmentStepper script (propose_step! for ops >= 0, step_success! for -1); returns one
+    dict per op with s, s1, ds, t, t1, dt, abs_delta, forward, done, dist."""
+    L = lib()
+    L.orc_segment_run.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                                           ctypes.POINTER(ctypes.c_double)]
+    start, target = complex(start), complex(target)
+    o = np.ascontiguousarray(ops, dtype=np.float64)
+    out = np.zeros(14 * len(o))
+    rc = L.orc_segment_run(start.real, start.imag, target.real, target.imag, len(o), _dp(o), _dp(out))
+    assert rc == 0
+    rows = []
+    for r in out.reshape(-1, 14):
+        rows.append(dict(s=r[0], s1=r[1], ds=r[2], t=complex(r[3], r[4]), t1=complex(r[5], r[6]),
+                         dt=complex(r[7], r[8]), abs_delta=r[9], forward=bool(r[10]), done=bool(r[11]), dist=r[12]))
+    return rows

@@ -712,8 +712,8 @@ static double norm_inf(const cplx* x, const double* w, int n) {
     for (int i = 1; i < n; i++) dmax = max_fast(dmax, w ? cabs2(cdivr(x[i], w[i])) : cabs2(x[i]));
     double d = sqrt(dmax);
     if (isinf(d)) {
-        dmax = w ? fast_abs(cdivr(x[0], w[0])) : fast_abs(x[0]);
-        for (int i = 1; i < n; i++) dmax = jmax(dmax, w ? fast_abs(cdivr(x[i], w[i])) : fast_abs(x[i]));
+        dmax = w ? chypot(cdivr(x[0], w[0])) : chypot(x[0]);
+        for (int i = 1; i < n; i++) dmax = jmax(dmax, w ? chypot(cdivr(x[i], w[i])) : chypot(x[i]));
         return dmax;
     }
     return d;
@@ -725,10 +725,10 @@ static double dist_inf(const cplx* x, const cplx* y, const double* w, int n) {
     if (isinf(d)) {
         /* note: the reference's unweighted fallback mixes abs and abs2 (src/norm.jl:160-163) */
         if (w) {
-            dmax = fast_abs(cdivr(csub(x[0], y[0]), w[0]));
-            for (int i = 1; i < n; i++) dmax = jmax(dmax, fast_abs(cdivr(csub(x[i], y[i]), w[i])));
+            dmax = chypot(cdivr(csub(x[0], y[0]), w[0]));
+            for (int i = 1; i < n; i++) dmax = jmax(dmax, chypot(cdivr(csub(x[i], y[i]), w[i])));
         } else {
-            dmax = fast_abs(csub(x[0], y[0]));
+            dmax = chypot(csub(x[0], y[0]));
             for (int i = 1; i < n; i++) dmax = jmax(dmax, cabs2(csub(x[i], y[i])));
         }
         return dmax;
@@ -850,7 +850,7 @@ static double inf_norm_ws(const orc_ws* W, const double* dl, const double* dr) {
 /* LA.cond(WS, d_l, d_r), src/linear_algebra.jl:745-774 */
 static double ws_cond(orc_ws* W, const double* dl, const double* dr) {
     if (W->n == 1) {
-        double a = sqrt(W->A[0].re * W->A[0].re + W->A[0].im * W->A[0].im);
+        double a = chypot(W->A[0]);
         if (!dl && !dr) return 1.0 / a;
         if (!dl) return 1.0 / (a * dr[0]);
         if (!dr) return 1.0 / (dl[0] * a);
@@ -995,8 +995,8 @@ static void tracker_destroy(orc_tracker* T) {
 static void seg_init(orc_tracker* T, cplx start, cplx target) {
     T->seg_start = start; T->seg_target = target;
     cplx d = csub(target, start);
-    T->seg_absD = hypot(d.re, d.im);
-    T->seg_forward = hypot(start.re, start.im) < hypot(target.re, target.im);
+    T->seg_absD = chypot(d);
+    T->seg_forward = chypot(start) < chypot(target);
     T->seg_s = T->seg_s1 = T->seg_forward ? 0.0 : T->seg_absD;
 }
 static int seg_is_done(const orc_tracker* T) { return T->seg_forward ? T->seg_s == T->seg_absD : T->seg_s == 0.0; }
@@ -2179,6 +2179,25 @@ double orc_norm(int n, const double* x, const double* w) {
     double r = norm_inf(xx, w, n);
     free(xx);
     return r;
+}
+/* SegmentStepper script for pinning (src/utils.jl:300-401): ops[k] < 0 is step_success!,
+ * otherwise propose_step!(ops[k]).  After each op 12 doubles are written:
+ * s, s', Δs, t, t', Δt (complex as re, im pairs), |Δ|, forward, is_done. */
+int orc_segment_run(double sre, double sim, double tre, double tim, int nops, const double* ops, double* out) {
+    orc_tracker* T = calloc(1, sizeof(orc_tracker));
+    if (!T) return -1;
+    seg_init(T, C(sre, sim), C(tre, tim));
+    for (int k = 0; k < nops; k++) {
+        if (ops[k] < 0) seg_step_success(T); else seg_propose(T, ops[k]);
+        double* r = out + 14 * k;
+        cplx t = seg_t(T), t1 = seg_t1(T), dt = seg_dt(T);
+        r[0] = T->seg_s; r[1] = T->seg_s1; r[2] = seg_ds(T);
+        r[3] = t.re; r[4] = t.im; r[5] = t1.re; r[6] = t1.im; r[7] = dt.re; r[8] = dt.im;
+        r[9] = T->seg_absD; r[10] = T->seg_forward; r[11] = seg_is_done(T);
+        r[12] = seg_dist_to_target(T); r[13] = 0;
+    }
+    free(T);
+    return 0;
 }
 double orc_nthroot(double x, int N) { return nthroot(x, N); }
 double orc_eps(double x) { return jeps(x); }

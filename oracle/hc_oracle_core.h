@@ -44,6 +44,19 @@ static inline cplx cmulsub(cplx a, cplx b, cplx c) {
 static inline double cabs2(cplx a) { return fma(a.re, a.re, a.im * a.im); }
 /* fast_abs, src/utils.jl:207 */
 static inline double fast_abs(cplx a) { return sqrt(cabs2(a)); }
+/* Base.abs(::Complex) = hypot(re, im): overflow-free.  Scaling by a power of two is exact, so
+ * this is deterministic (IEEE-exact operations only) and within one ulp. */
+static inline double chypot(cplx a) {
+    double x = fabs(a.re), y = fabs(a.im);
+    if (isinf(x) || isinf(y)) return INFINITY;
+    if (isnan(x) || isnan(y)) return NAN;
+    double m = x > y ? x : y;
+    if (m == 0) return 0.0;
+    int e;
+    frexp(m, &e);
+    double xs = ldexp(x, -e), ys = ldexp(y, -e);
+    return ldexp(sqrt(fma(xs, xs, ys * ys)), e);
+}
 static inline cplx cscale(double s, cplx a) { return C(s * a.re, s * a.im); }
 static inline cplx cdivr(cplx a, double s) { return C(a.re / s, a.im / s); }
 static inline cplx cdiv(cplx a, cplx b) {
