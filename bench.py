@@ -27,6 +27,16 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes of this workload,
+    each scaled by what the counter reads for a plain copy of known size; None if not profiled."""
+    try:
+        t = json.loads((REPO / "profiles" / "traffic.json").read_text())[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+    return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0)
+
+
 def build_workload(learned: bool):
     from hcgpu.configs import dense_system
     from hcgpu.homotopies import total_degree
@@ -143,7 +153,7 @@ def main():
                    "n": n, "paths": npaths, "system_seed": 1, "gamma_seed": 2, "tracker": "plain (adaptive rule)"},
         "roofline": {"bound": "mfma", "bound_detail": "FP64 compute roof (MI355X FP64 vector = matrix peak)",
                      "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": measured_traffic("dense_n8_6561paths"),
                      "kernel": "hcg_track_kernel", "kernel_ms": round(mean_ms, 3),
                      "flops_per_launch": mean_flops, "flops_per_step": mean_flops / max(1, np.mean(steps))},
         "steps_per_path": round(float(np.mean(steps)) / npaths, 3),
