@@ -104,6 +104,7 @@ struct Lane {
 // tp!, src/homotopies/parameter_homotopy.jl:66-87
 __device__ void H_tp(Lane& L, cplx t) {
     if (L.tcache_valid && L.tcache.re == t.re && L.tcache.im == t.im) return;
+    L.flops += (t.im == 0.0 ? 8.0 : 16.0) * HCG_NPAR;
     for (int i = 0; i < HCG_NPAR; i++) {
         cplx p = mkc(L.p[2 * i], L.p[2 * i + 1]), q = mkc(L.q[2 * i], L.q[2 * i + 1]);
         cplx pti;

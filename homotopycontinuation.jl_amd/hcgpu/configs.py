@@ -86,3 +86,16 @@ def shared_factor_system(n: int, seed: int = 0) -> System:
 def parameter_draws(nparams: int, count: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return cgauss(rng, count, nparams)
+
+
+# The batched parameter-homotopy workload: n = 10 sparse quadrics (20 monomials each) with
+# coefficients affine in 16 parameters; 1,024 start solutions at p0 tracked to 256 targets.
+CFG2_SEEDS = dict(system=20, p0=21, targets=22, gamma=23)
+
+
+def cfg2_problem(ntargets: int = 256):
+    """(F, p0, Q): the system, its start parameters and `ntargets` target parameter vectors."""
+    F = sparse_param_system(10, 20, 16, seed=CFG2_SEEDS["system"])
+    p0 = parameter_draws(16, 1, seed=CFG2_SEEDS["p0"])[0]
+    Q = parameter_draws(16, 256, seed=CFG2_SEEDS["targets"])[:ntargets]
+    return F, p0, Q

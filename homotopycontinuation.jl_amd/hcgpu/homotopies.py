@@ -74,21 +74,87 @@ class StraightLineHomotopy(AbstractHomotopy):
         return self._desc
 
 
+def affine_parameter_lift(F: System):
+    """Lift a system whose coefficients are affine in its parameters to the system in its
+    coefficients: F(x; p) = F̃(x; c(p)) with F̃_i = Σ_α c_{iα} x^α and c(p) affine.
+
+    Because c is affine, c(t p + (1 - t) q) = t c(p) + (1 - t) c(q): the parameter homotopy
+    of F from p to q and the one of F̃ from c(p) to c(q) follow the same path x(t).  F̃ is
+    much cheaper to evaluate (one interpolation per coefficient instead of re-forming every
+    coefficient from all parameters).  Returns (F̃, c) with c(p) evaluated in the same order
+    of operations as System.fix_parameters, so c(p) reproduces F(x; p)'s coefficients bit
+    for bit."""
+    nv = F.nvars
+    if hasattr(F, "exprs"):
+        raise NotImplementedError("lift the expanded polynomial system")
+    index, slots, maps = {}, [], []
+    for i, f in enumerate(F.polys):
+        for e, c in f.terms.items():
+            pe = e[nv:]
+            if sum(pe) > 1:
+                raise ValueError("coefficients are not affine in the parameters")
+            key = (i, e[:nv])
+            if key not in index:
+                index[key] = len(slots)
+                slots.append(key)
+                maps.append([])
+            maps[index[key]].append((pe.index(1) if sum(pe) == 1 else -1, c))
+    m = len(slots)
+    terms = [dict() for _ in F.polys]
+    for s, (i, alpha) in enumerate(slots):
+        e = list(alpha) + [0] * m
+        e[nv + s] = 1
+        terms[i][tuple(e)] = 1.0
+    Ft = System([Poly(t, nv + m) for t in terms], nv, m)
+
+    def cmap(p):
+        p = np.asarray(p, dtype=np.complex128)
+        out = np.empty(m, dtype=np.complex128)
+        for s, lst in enumerate(maps):
+            acc = 0j
+            for k, c in lst:
+                acc = acc + (complex(c) if k < 0 else complex(c) * complex(p[k]))
+            out[s] = acc
+        return out
+
+    return Ft, cmap
+
+
 class ParameterHomotopy(AbstractHomotopy):
     """H(x, t) = F(x; t p + (1 - t) q).  `q` may be one vector or a list of target
-    vectors (one group per vector, as many_solve's target_parameters)."""
+    vectors (one group per vector, as many_solve's target_parameters).
 
-    def __init__(self, F: System, start_parameters: Sequence[complex], target_parameters):
+    lift_affine=True tracks the same homotopy through affine_parameter_lift (for systems
+    whose coefficients are affine in the parameters); p and q are then mapped to
+    coefficient vectors and `self.F` holds the lifted system."""
+
+    def __init__(self, F: System, start_parameters: Sequence[complex], target_parameters, lift_affine: bool = False):
         if F.nparams == 0:
             raise ValueError("parameter homotopy needs a system with parameters")
+        q = np.asarray(target_parameters, dtype=np.complex128).reshape(-1, F.nparams)
+        p = np.asarray(start_parameters, dtype=np.complex128)
+        if p.shape != (F.nparams,):
+            raise ValueError("start parameters have the wrong length")
+        self.original = F
+        self.lifted = bool(lift_affine)
+        self._cmap = None
+        if lift_affine:
+            F, cmap = affine_parameter_lift(F)
+            self._cmap = cmap
+            p = cmap(p)
+            q = np.stack([cmap(qq) for qq in q])
         self.F = _SysTapes(F)
         self.n = F.nvars
-        self.p = np.asarray(start_parameters, dtype=np.complex128)
-        q = np.asarray(target_parameters, dtype=np.complex128)
-        self.q = q.reshape(-1, F.nparams)
-        if self.p.shape != (F.nparams,):
-            raise ValueError("start parameters have the wrong length")
+        self.p = p
+        self.q = q
         self._desc = None
+
+    def target_matrix(self, target_parameters) -> np.ndarray:
+        """Target parameter vectors (one per row) in the parameter space the tapes use."""
+        Q = np.asarray(target_parameters, dtype=np.complex128).reshape(-1, self.original.nparams)
+        if self._cmap is None:
+            return Q
+        return np.stack([self._cmap(q) for q in Q])
 
     def desc(self) -> hcg_homotopy_desc:
         if self._desc is None:

@@ -90,7 +90,8 @@ class Solver:
             out = self.tracker.track_batch(starts)
             return Result(path_results_from_batch(out, starts), seed=self.seed, start_system=self.start_system)
         # many parameters: every (target, start) pair in one launch
-        Q = np.asarray(target_parameters, dtype=np.complex128).reshape(-1, len(H.p))
+        Qorig = np.asarray(target_parameters, dtype=np.complex128).reshape(-1, H.original.nparams)
+        Q = H.target_matrix(Qorig)
         S = len(starts)
         all_starts = np.tile(starts, (len(Q), 1))
         groups = np.repeat(np.arange(len(Q), dtype=np.int32), S)
@@ -101,13 +102,13 @@ class Solver:
             chunk = prs[g * S:(g + 1) * S]
             for k, r in enumerate(chunk):
                 r.path_number = k + 1
-            results.append((Result(chunk, seed=self.seed), Q[g]))
+            results.append((Result(chunk, seed=self.seed), Qorig[g]))
         return results
 
 
 def solver_startsolutions(F: System, starts=None, *, start_system="total_degree", gamma=None, seed=None,
                           start_parameters=None, target_parameters=None, tracker_options=None,
-                          endgame_options=None, **gpu):
+                          endgame_options=None, lift_affine=False, **gpu):
     if seed is None:
         seed = int(np.random.default_rng().integers(0, 2**32))
     if start_parameters is not None:
@@ -115,7 +116,7 @@ def solver_startsolutions(F: System, starts=None, *, start_system="total_degree"
             raise ValueError("target_parameters required")
         q = np.asarray(target_parameters, dtype=np.complex128)
         q0 = q if q.ndim == 1 else q[0]
-        H = ParameterHomotopy(F, start_parameters, q0)
+        H = ParameterHomotopy(F, start_parameters, q0, lift_affine=lift_affine)
         T = EndgameTracker(H, tracker_options, endgame_options, **gpu)
         return Solver(T, seed=seed), starts
     if start_system != "total_degree":
