@@ -91,7 +91,9 @@ struct Lane {
     double sing_t;
     // learned proposer bookkeeping
     int learned_proposed, learned_accepted, last_learned;
-    double last_theta, cond_H_xd;
+    double cond_H_xd, ds_plain;
+    float feat[8], inf_logit;
+    const unsigned short* mlp_w;  // BF16 weights in LDS, or null (plain rule only)
     double flops;  // counted FP64 flops of this path
     // step log
     float* log_records;
@@ -920,38 +922,30 @@ __device__ __noinline__ double refine_current_solution(Lane& L, double min_tol, 
     return mu;
 }
 
-__device__ __noinline__ void log_step(Lane& L, cplx t, double ds, const NewtRes& res, double le, double tau,
-                                     double omega, double mu, int acc) {
+__device__ __noinline__ void log_step(Lane& L, const NewtRes& res, int acc) {
     int idx = atomicAdd(L.log_count, 1);
     if (idx >= L.log_capacity) return;
     float* r = L.log_records + (size_t)idx * HCG_STEP_RECORD_FLOATS;
     r[0] = (float)L.path;
     r[1] = (float)(L.accepted + L.rejected - 1);
-    r[2] = (float)t.re;
-    r[3] = (float)hc_log(ds);
-    r[4] = (float)acc;
-    r[5] = (float)res.iters;
-    r[6] = (float)res.theta;
-    r[7] = (float)hc_log(le * ipow(ds, 4));
-    r[8] = (float)(hc_log(L.cond_H_xd) / 2.302585092994046);
-    r[9] = (float)hc_log(omega);
-    r[10] = (float)hc_log(mu);
-    r[11] = (float)hc_log(tau);
+    for (int i = 0; i < HCG_FEATS; i++) r[2 + i] = L.feat[i];
+    r[10] = (float)acc;
+    r[11] = (float)res.iters;
     r[12] = (float)hc_log(seg_ds(L));
     r[13] = (float)(L.extended_prec + 2 * L.singular_endgame + 4 * L.last_learned);
-    r[14] = (float)(hc_log(L.accuracy) / 2.302585092994046);
-    r[15] = (float)L.steps_eg;
+    r[14] = (float)(hc_log(L.accuracy) / HCG_LN10);
+    r[15] = (float)(isnan(L.ds_plain) ? hc_log(seg_ds(L)) : hc_log(L.ds_plain));
 }
 
 // step!(tracker), src/tracker.jl:851-926; returns !last_step_failed
 __device__ __noinline__ int tracker_step(Lane& L) {
     const hcg_options* o = L.o;
     cplx t = seg_t(L), dt = seg_dt(L), t1 = seg_t1(L);
-    const double ds0 = seg_ds(L), le0 = L.local_error, tau0 = L.tau, om0 = L.omega, mu0 = L.mu;
+    const double ds0 = seg_ds(L);
     predict(L, t, dt);
     wnorm_update(L, L.xh);
     NewtRes res = newton(L, L.xh, t1, L.mu, L.omega, L.extended_prec, 0, L.accepted == 0);
-    learned_record(L, res);
+    if (res.code == NEWT_CONVERGED && L.last_learned) L.learned_accepted++;
     if (res.code == NEWT_CONVERGED) {
         for (int i = 0; i < NN; i++) L.x[i] = L.xb[i];
         L.ds_prev = seg_ds(L);
@@ -975,8 +969,11 @@ __device__ __noinline__ int tracker_step(Lane& L) {
         L.ext_rej += L.extended_prec;
         L.last_failed++;
     }
+    if (L.log_records || L.mlp_w) step_features(L, res, ds0);
+    L.last_learned = 0;
+    L.ds_plain = HCG_NAN;
     update_stepsize(L, res);
-    if (L.log_records) log_step(L, t, ds0, res, le0, tau0, om0, mu0, res.code == NEWT_CONVERGED);
+    if (L.log_records) log_step(L, res, res.code == NEWT_CONVERGED);
     check_terminated(L);
     return !(L.last_failed > 0);
 }
@@ -1088,7 +1085,9 @@ __device__ __noinline__ void eg_init(Lane& L, double t1) {
     L.refined_ext = 0;
     L.learned_proposed = L.learned_accepted = 0;
     L.last_learned = 0;
-    L.last_theta = HCG_NAN;
+    L.inf_logit = 0.0f;
+    L.ds_plain = HCG_NAN;
+    for (int i = 0; i < 8; i++) L.feat[i] = 0.0f;
 #if HCG_KIND == 1
     L.tcache_valid = 0;
 #endif
@@ -1396,8 +1395,15 @@ __device__ __noinline__ void write_result_invalid(Lane& L, const hcg_kargs& a, i
 }
 
 extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
+    __shared__ unsigned short mlp_lds[HCG_MLP_NPARAMS];
     Lane L;
     L.o = a.opts;
+    L.mlp_w = nullptr;
+    if (a.mlp && a.opts->learned_step) {  // uniform over the launch
+        for (int i = threadIdx.x; i < HCG_MLP_NPARAMS; i += blockDim.x) mlp_lds[i] = f32_to_bf16(a.mlp[i]);
+        __syncthreads();
+        L.mlp_w = mlp_lds;
+    }
     L.log_records = a.log_records;
     L.log_count = a.log_count;
     L.log_capacity = a.log_capacity;
@@ -1405,7 +1411,7 @@ extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
     int phase = 0;  // 0: fetch, 1: track
     int k = -1;
     int guard = 0;  // loop iterations of the current path (each performs >= 1 tracker step)
-    long long my_steps = 0, my_newton = 0;
+    long long my_steps = 0, my_newton = 0, my_lacc = 0;
     double my_flops = 0.0;
     while (true) {
         if (phase == 0) {
@@ -1438,6 +1444,7 @@ extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
             write_result(L, a, k);
             my_steps += L.accepted + L.rejected;
             my_newton += L.newton_iters;
+            my_lacc += L.learned_accepted;
             my_flops += L.flops;
             phase = 0;
         }
@@ -1445,5 +1452,6 @@ extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
     atomicAdd(&a.counters[0], (unsigned long long)my_steps);
     atomicAdd(&a.counters[1], (unsigned long long)my_newton);
     atomicAdd(&a.counters[2], (unsigned long long)my_flops);
+    atomicAdd(&a.counters[3], (unsigned long long)my_lacc);
 }
 

@@ -131,6 +131,7 @@ extern "C" void hcg_default_options(hcg_options* o) {
     o->val_finite_tol = 0.05; o->val_at_infinity_tol = 0.01;
     o->sing_cond = 1e14; o->sing_accuracy = 1e-12; o->scaling_threshold = -30.0;
     o->scale_min = 1e-4; o->scale_abs_min = 1e-6; o->scale_max = std::ldexp(1.0, 511);
+    o->learned_max_mult = 4.0; o->learned_min_t = 0.0;
 }
 
 // ------------------------------ hipRTC ------------------------------
@@ -322,6 +323,8 @@ static int validate_batch_args(hcg_ctx* ctx, const hcg_program* prog, const hcg_
     if (opts->max_winding_number < 1 || opts->min_newton_iters < 0 || opts->min_newton_iters > 20 || opts->max_steps < 0)
         return fail(HCG_ERR_INVALID_ARGUMENT, "invalid options");
     if (opts->learned_step && !ctx->mlp_loaded) return fail(HCG_ERR_INVALID_ARGUMENT, "learned_step requested but no MLP loaded");
+    if (opts->learned_step && !(opts->learned_max_mult >= 1.0))
+        return fail(HCG_ERR_INVALID_ARGUMENT, "learned_max_mult must be >= 1");
     return HCG_OK;
 }
 

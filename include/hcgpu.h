@@ -122,6 +122,8 @@ typedef struct hcg_options {
     int32_t block_size;       /* 0: autotuned */
     int32_t blocks_per_cu;    /* 0: autotuned */
     int32_t log_steps;        /* 1: write per-step feature records into hcg_step_log */
+    double learned_max_mult;  /* largest factor a learned proposal may lengthen the plain one by */
+    double learned_min_t;     /* learned proposals only while |t| > learned_min_t */
 } hcg_options;
 
 /* A batch of paths for one program. */

@@ -71,6 +71,7 @@ typedef struct orc_options {
     double sing_cond, sing_accuracy, scaling_threshold;
     double scale_min, scale_abs_min, scale_max;
     int32_t learned_step, block_size, blocks_per_cu, log_steps;
+    double learned_max_mult, learned_min_t;
 } orc_options;
 typedef struct orc_batch {
     int32_t npaths, ngroups;
@@ -1514,27 +1515,37 @@ static double refine_current_solution(orc_tracker* T, double min_tol, int nsteps
     return mu;
 }
 
-static void log_step(orc_tracker* T, cplx t, double ds, const newton_result* res, double le, double tau, double omega,
-                     double mu, int acc) {
+/* Per-step features of the learned step-size proposer's training data (the GPU tracker
+ * forms the same eight values after every step): log Δs, Newton contraction θ,
+ * log(e Δs^4), t, log10 cond(H_x, ẋ), log ω, log μ, log τ; NaN -> 0, clamped to ±80. */
+static float feat_clean(double v) { return isnan(v) ? 0.0f : (float)jclamp(v, -80.0, 80.0); }
+static void step_features(orc_tracker* T, const newton_result* res, double ds0, float* f) {
+    f[0] = feat_clean(hc_log(ds0));
+    f[1] = feat_clean(res->theta);
+    f[2] = feat_clean(hc_log(T->local_error * powi(ds0, 4)));
+    f[3] = feat_clean(seg_t(T).re);
+    f[4] = feat_clean(hc_log(T->cond_H_xd) / 2.302585092994046);
+    f[5] = feat_clean(hc_log(T->omega));
+    f[6] = feat_clean(hc_log(T->mu));
+    f[7] = feat_clean(hc_log(T->tau));
+}
+
+/* step record: 0 path, 1 step index, 2-9 features, 10 accepted, 11 Newton iterations,
+ * 12 log of the next proposed Δs, 13 flags (1 extended precision, 2 singular endgame),
+ * 14 log10 accuracy, 15 log of the plain rule's next proposal (same as 12 here) */
+static void log_step(orc_tracker* T, const float* feat, const newton_result* res, int acc) {
     int idx = atomic_fetch_add(T->log_count, 1);
     if (idx >= T->log_capacity) return;
     float* r = T->log_records + (size_t)idx * 16;
     r[0] = (float)T->path;
     r[1] = (float)(T->accepted + T->rejected - 1);
-    r[2] = (float)t.re;
-    r[3] = (float)hc_log(ds);
-    r[4] = (float)acc;
-    r[5] = (float)res->iters;
-    r[6] = (float)res->theta;
-    r[7] = (float)hc_log(le * powi(ds, 4));
-    r[8] = (float)(hc_log(T->cond_H_xd) / 2.302585092994046);
-    r[9] = (float)hc_log(omega);
-    r[10] = (float)hc_log(mu);
-    r[11] = (float)hc_log(tau);
+    for (int i = 0; i < 8; i++) r[2 + i] = feat[i];
+    r[10] = (float)acc;
+    r[11] = (float)res->iters;
     r[12] = (float)hc_log(seg_ds(T));
     r[13] = (float)(T->extended_prec + 2 * T->singular_endgame);
     r[14] = (float)(hc_log(T->accuracy) / 2.302585092994046);
-    r[15] = (float)T->steps_eg;
+    r[15] = r[12];
 }
 
 /* step!(tracker), src/tracker.jl:851-926; returns !last_step_failed */
@@ -1542,7 +1553,7 @@ static int tracker_step(orc_tracker* T) {
     const orc_options* o = T->opt;
     int n = T->n;
     cplx t = seg_t(T), dt = seg_dt(T), t1 = seg_t1(T);
-    const double ds0 = seg_ds(T), le0 = T->local_error, tau0 = T->tau, om0 = T->omega, mu0 = T->mu;
+    const double ds0 = seg_ds(T);
     predict(T, T->xh, T->x, t, dt);
     wnorm_update(T, T->xh);
     newton_result res = newton(T, T->xb, T->xh, t1, T->mu, T->omega, T->extended_prec, 0, T->accepted == 0);
@@ -1570,8 +1581,10 @@ static int tracker_step(orc_tracker* T) {
         T->last_steps_failed++;
     }
     T->norm_dx0 = res.norm_dx0;
+    float feat[8];
+    if (T->log_records) step_features(T, &res, ds0, feat);
     update_stepsize(T, &res);
-    if (T->log_records) log_step(T, t, ds0, &res, le0, tau0, om0, mu0, res.code == NEWT_CONVERGED);
+    if (T->log_records) log_step(T, feat, &res, res.code == NEWT_CONVERGED);
     check_terminated(T);
     return !(T->last_steps_failed > 0);
 }
