@@ -498,7 +498,9 @@ static void gen_system(std::ostringstream& os, const hcg_system& s, const std::s
     }
 }
 
-bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, std::string* err) {
+// Validates one homotopy and writes its HCG_* macros, system functions and static flop
+// counts (everything a tracker body needs) to `os`.
+static bool emit_system(std::ostringstream& os, const hcg_homotopy_desc* h, Flops* fl, std::string* err) {
     int n = h->n;
     if (n < 1 || n > HCG_MAX_N) { *err = "system size out of range (1.." + std::to_string(HCG_MAX_N) + ")"; return false; }
     if (h->kind != HCG_STRAIGHT_LINE && h->kind != HCG_PARAMETER) { *err = "unknown homotopy kind"; return false; }
@@ -514,12 +516,9 @@ bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, st
         return false;
     }
     *fl = Flops();
-    std::ostringstream os;
-    os << "// generated by hcgpu codegen\n";
     os << "#define HCG_N " << n << "\n#define HCG_KIND " << h->kind << "\n";
     os << "#define HCG_NPAR " << (h->kind == HCG_PARAMETER ? h->target.eval->nparams : 0) << "\n";
     os << "#define HCG_GAMMA_RE " << dlit(h->gamma_re) << "\n#define HCG_GAMMA_IM " << dlit(h->gamma_im) << "\n";
-    os << "#include \"hcg_arith.h\"\n";
     if (h->kind == HCG_STRAIGHT_LINE) {
         gen_system(os, h->start, "sysG", n, h->start.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
         gen_system(os, h->target, "sysF", n, h->target.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
@@ -541,7 +540,85 @@ bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, st
     os << "#define HCG_FL_T1 " << (h->kind == HCG_STRAIGHT_LINE ? fl->eval + comb_t1 : fl->taylor[1]) << "\n";
     os << "#define HCG_FL_T2 " << fl->taylor[2] + comb_t23 << "\n";
     os << "#define HCG_FL_T3 " << fl->taylor[3] + comb_t23 << "\n";
+    return true;
+}
+
+bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, std::string* err) {
+    std::ostringstream os;
+    os << "// generated by hcgpu codegen\n";
+    os << "#include \"hcg_arith.h\"\n";
+    if (!emit_system(os, h, fl, err)) return false;
     os << "#include \"hcg_tracker.h\"\n";
+    *src = os.str();
+    return true;
+}
+
+// Several homotopies in one program: system s lives in namespace hcs<s> (its functions
+// plus its own instance of the tracker body); one persistent kernel takes paths from a
+// single work queue over all systems and dispatches each to its system's tracker.
+bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::string* src, Flops* fls,
+                           std::string* err) {
+    if (nsys < 1 || nsys > 8) { *err = "between 1 and 8 systems per program"; return false; }
+    std::ostringstream os;
+    os << "// generated by hcgpu codegen (multi-system program)\n";
+    os << "#include \"hcg_arith.h\"\n#include \"hcgpu.h\"\n#include \"hcg_kargs.h\"\n#include \"hcg_mlp_infer.h\"\n";
+    for (int s = 0; s < nsys; s++) {
+        os << "namespace hcs" << s << " {\n";
+        std::string e;
+        if (!emit_system(os, hs[s], &fls[s], &e)) { *err = "system " + std::to_string(s) + ": " + e; return false; }
+        os << "#include \"hcg_tracker_body.h\"\n";
+        for (const char* m : {"HCG_N", "HCG_KIND", "HCG_NPAR", "HCG_GAMMA_RE", "HCG_GAMMA_IM", "HCG_FL_EVAL", "HCG_FL_EJ",
+                              "HCG_FL_DD", "HCG_FL_T1", "HCG_FL_T2", "HCG_FL_T3"})
+            os << "#undef " << m << "\n";
+        os << "}  // namespace hcs" << s << "\n";
+    }
+    os << "union hcg_lane_union {\n";
+    for (int s = 0; s < nsys; s++) os << "    hcs" << s << "::Lane l" << s << ";\n";
+    os << "};\n";
+    auto each = [&](const std::string& body) {
+        std::string out = "            switch (s) {\n";
+        for (int s = 0; s < nsys; s++) {
+            std::string b = body;
+            std::string ns = "hcs" + std::to_string(s), l = "U.l" + std::to_string(s), a = "m.sys[" + std::to_string(s) + "]";
+            for (size_t p; (p = b.find("@NS")) != std::string::npos;) b.replace(p, 3, ns);
+            for (size_t p; (p = b.find("@L")) != std::string::npos;) b.replace(p, 2, l);
+            for (size_t p; (p = b.find("@A")) != std::string::npos;) b.replace(p, 2, a);
+            out += "            case " + std::to_string(s) + ": " + b + " break;\n";
+        }
+        return out + "            }\n";
+    };
+    os << R"(extern "C" __global__ void __launch_bounds__(64) hcg_track_multi_kernel(hcg_multi_kargs m) {
+    __shared__ unsigned short mlp_lds[HCG_MLP_NPARAMS];
+    const unsigned short* mw = stage_mlp(m.sys[0].mlp, m.sys[0].opts, mlp_lds);
+    hcg_lane_union U;
+    LaneTotals T = {0, 0, 0, 0.0};
+    int phase = 0, k = -1, s = 0;
+    while (true) {
+        int ended = 0;
+        if (phase == 0) {
+            int kg = atomicAdd(m.next_path, 1);
+            if (kg >= m.total) break;
+            s = 0;
+            while (s + 1 < m.nsys && kg >= m.offset[s + 1]) s++;
+            k = kg - m.offset[s];
+            int live = 0;
+)" << each("{ @NS::lane_setup(@L, @A, mw); live = @NS::lane_begin(@L, @A, k); if (live) ended = @NS::lane_ended(@L); }")
+       << R"(            if (!live) continue;
+            phase = 1;
+        } else {
+)" << each("{ @NS::lane_iter(@L); ended = @NS::lane_ended(@L); }")
+       << R"(        }
+        if (ended) {
+)" << each("@NS::lane_finish(@L, @A, k, T);")
+       << R"(            phase = 0;
+        }
+    }
+    atomicAdd(&m.sys[0].counters[0], (unsigned long long)T.steps);
+    atomicAdd(&m.sys[0].counters[1], (unsigned long long)T.newton);
+    atomicAdd(&m.sys[0].counters[2], (unsigned long long)T.flops);
+    atomicAdd(&m.sys[0].counters[3], (unsigned long long)T.lacc);
+}
+)";
     *src = os.str();
     return true;
 }

@@ -24,3 +24,19 @@ typedef struct hcg_kargs {
     int32_t* log_count;
     const float* mlp;              // learned proposer weights (FP32 master copy) or NULL
 } hcg_kargs;
+
+// Per-lane totals, added to the launch counters when the lane exits.
+typedef struct LaneTotals {
+    long long steps, newton, lacc;
+    double flops;
+} LaneTotals;
+
+// Several systems in one launch: paths [offset[s], offset[s+1]) belong to system s,
+// whose inputs and outputs are described by sys[s]; one work queue over all of them.
+#define HCG_MAX_SYSTEMS 8
+typedef struct hcg_multi_kargs {
+    int32_t nsys, total;
+    int32_t offset[HCG_MAX_SYSTEMS + 1];
+    int32_t* next_path;
+    hcg_kargs sys[HCG_MAX_SYSTEMS];
+} hcg_multi_kargs;

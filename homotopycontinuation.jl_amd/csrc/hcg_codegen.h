@@ -16,5 +16,7 @@ struct Flops {
 
 bool validate_tape(const hcg_tape* tp, int n, std::string* err);
 bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, std::string* err);
+bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::string* src, Flops* fls,
+                           std::string* err);
 
 }  // namespace hcg

@@ -18,6 +18,7 @@
 #include <fstream>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -70,6 +71,9 @@ struct hcg_program {
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;
     hcg::Flops flops;
+    // multi-system programs (hcg_program_create_multi): per-system size, kind, parameters
+    int nsys = 0;
+    std::vector<int> ns, kinds, npars;
 };
 
 extern "C" const char* hcg_last_error(void) { return g_err.c_str(); }
@@ -286,6 +290,58 @@ extern "C" int hcg_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* h, hcg_
     return HCG_OK;
 }
 
+static int multi_descs_ok(const hcg_homotopy_desc* const* hs, int32_t nsys) {
+    if (!hs || nsys < 1 || nsys > 8) return fail(HCG_ERR_INVALID_ARGUMENT, "between 1 and 8 homotopies per program");
+    for (int s = 0; s < nsys; s++)
+        if (!hs[s]) return fail(HCG_ERR_INVALID_ARGUMENT, "homotopy %d is NULL", s);
+    return HCG_OK;
+}
+
+extern "C" int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, size_t* code_size) {
+    int rc = multi_descs_ok(hs, nsys);
+    if (rc) return rc;
+    std::string src, err;
+    std::vector<hcg::Flops> fl(nsys);
+    if (!hcg::generate_multi_source(hs, nsys, &src, fl.data(), &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    std::vector<char> code;
+    rc = rtc_compile(src, &code);
+    if (rc) return rc;
+    if (code_size) *code_size = code.size();
+    return HCG_OK;
+}
+
+extern "C" int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys,
+                                        hcg_program** out) {
+    if (!ctx || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = multi_descs_ok(hs, nsys);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hcg_program* p = new hcg_program();
+    p->ctx = ctx;
+    p->nsys = nsys;
+    for (int s = 0; s < nsys; s++) {
+        p->ns.push_back(hs[s]->n);
+        p->kinds.push_back(hs[s]->kind);
+        p->npars.push_back(hs[s]->kind == HCG_PARAMETER && hs[s]->target.eval ? hs[s]->target.eval->nparams : 0);
+    }
+    p->n = *std::max_element(p->ns.begin(), p->ns.end());
+    std::string err;
+    std::vector<hcg::Flops> fl(nsys);
+    if (!hcg::generate_multi_source(hs, nsys, &p->source, fl.data(), &err)) {
+        delete p;
+        return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    }
+    std::vector<char> code;
+    rc = rtc_compile(p->source, &code);
+    if (rc) { delete p; return rc; }
+    hipError_t e = hipModuleLoadData(&p->module, code.data());
+    if (e != hipSuccess) { delete p; return fail(HCG_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e)); }
+    e = hipModuleGetFunction(&p->kernel, p->module, "hcg_track_multi_kernel");
+    if (e != hipSuccess) { hipModuleUnload(p->module); delete p; return fail(HCG_ERR_HIP, "hipModuleGetFunction: %s", hipGetErrorString(e)); }
+    *out = p;
+    return HCG_OK;
+}
+
 extern "C" void hcg_program_destroy(hcg_program* p) {
     if (!p) return;
     if (p->module) { hipSetDevice(p->ctx->device); hipModuleUnload(p->module); }
@@ -308,6 +364,7 @@ static int validate_batch_args(hcg_ctx* ctx, const hcg_program* prog, const hcg_
                                bool host) {
     if (!ctx || !prog || !b || !opts) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
+    if (prog->nsys) return fail(HCG_ERR_INVALID_ARGUMENT, "multi-system program: use hcg_track_multi");
     const long np = b->npaths;
     if (np < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "negative path count");
     if (np > (1L << 30) / prog->n) return fail(HCG_ERR_INVALID_ARGUMENT, "batch too large");
@@ -462,6 +519,123 @@ extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, con
     rc = launch_track(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, nullptr,
                       0, nullptr);
     if (rc) return rc;
+    return finish_launch(ctx);
+}
+
+extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches,
+                               const hcg_options* opts, hcg_path_result* results, double* solutions,
+                               double* last_points, double* valuations) {
+    if (!ctx || !prog || !batches || !opts || !results || !solutions || !last_points || !valuations)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
+    if (!prog->nsys) return fail(HCG_ERR_INVALID_ARGUMENT, "single-system program: use hcg_track_batch");
+    if (opts->max_winding_number < 1 || opts->min_newton_iters < 0 || opts->min_newton_iters > 20 || opts->max_steps < 0)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "invalid options");
+    if (opts->learned_step && !ctx->mlp_loaded) return fail(HCG_ERR_INVALID_ARGUMENT, "learned_step requested but no MLP loaded");
+    if (opts->learned_step && !(opts->learned_max_mult >= 1.0)) return fail(HCG_ERR_INVALID_ARGUMENT, "learned_max_mult must be >= 1");
+    const int S = prog->nsys;
+    long total = 0, tot_nx = 0;
+    for (int s = 0; s < S; s++) {
+        const hcg_batch* b = &batches[s];
+        if (b->npaths < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "system %d: negative path count", s);
+        if (b->npaths > 0 && !b->starts) return fail(HCG_ERR_INVALID_ARGUMENT, "system %d: starts is NULL", s);
+        if (prog->kinds[s] == HCG_PARAMETER && b->npaths > 0) {
+            if (!b->start_params || !b->target_params || b->ngroups < 1)
+                return fail(HCG_ERR_INVALID_ARGUMENT, "system %d: parameter homotopy needs parameters", s);
+            if (b->group_of_path)
+                for (long k = 0; k < b->npaths; k++)
+                    if (b->group_of_path[k] < 0 || b->group_of_path[k] >= b->ngroups)
+                        return fail(HCG_ERR_INVALID_ARGUMENT, "system %d: group_of_path[%ld] out of range", s, k);
+        }
+        total += b->npaths;
+        tot_nx += (long)b->npaths * prog->ns[s];
+    }
+    if (total > (1L << 30) || tot_nx > (1L << 31)) return fail(HCG_ERR_INVALID_ARGUMENT, "batch too large");
+    if (total == 0) return HCG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ctx_scalars(ctx);
+    if (rc) return rc;
+    // one allocation per array kind, sliced per system
+    DevBuf d_starts, d_res, d_sol, d_last, d_val, d_par;
+    HIPCHK(hipMalloc(&d_starts.p, sizeof(double) * 2 * tot_nx));
+    HIPCHK(hipMalloc(&d_res.p, sizeof(hcg_path_result) * total));
+    HIPCHK(hipMalloc(&d_sol.p, sizeof(double) * 2 * tot_nx));
+    HIPCHK(hipMalloc(&d_last.p, sizeof(double) * 2 * tot_nx));
+    HIPCHK(hipMalloc(&d_val.p, sizeof(double) * tot_nx));
+    long par_doubles = 0, grp_ints = 0;
+    for (int s = 0; s < S; s++)
+        if (prog->kinds[s] == HCG_PARAMETER && batches[s].npaths > 0) {
+            par_doubles += 2L * prog->npars[s] * (1 + batches[s].ngroups);
+            if (batches[s].group_of_path) grp_ints += batches[s].npaths;
+        }
+    if (par_doubles + grp_ints > 0) HIPCHK(hipMalloc(&d_par.p, sizeof(double) * (par_doubles + grp_ints)));
+    HIPCHK(hipMemcpyAsync(ctx->d_opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(int32_t), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * 4, ctx->stream));
+    hcg_multi_kargs m;
+    std::memset(&m, 0, sizeof m);
+    m.nsys = S;
+    m.total = (int32_t)total;
+    m.next_path = ctx->d_next;
+    long off = 0, offx = 0;
+    double* parp = (double*)d_par.p;
+    for (int s = 0; s < S; s++) {
+        const hcg_batch* b = &batches[s];
+        const int n = prog->ns[s];
+        const long np = b->npaths;
+        m.offset[s] = (int32_t)off;
+        hcg_kargs& ka = m.sys[s];
+        double* st = (double*)d_starts.p + 2 * offx;
+        if (np > 0) HIPCHK(hipMemcpyAsync(st, b->starts, sizeof(double) * 2 * n * np, hipMemcpyHostToDevice, ctx->stream));
+        ka.npaths = (int32_t)np;
+        ka.ngroups = b->ngroups;
+        ka.starts = st;
+        ka.t1 = b->t1;
+        ka.t0 = b->t0;
+        if (prog->kinds[s] == HCG_PARAMETER && np > 0) {
+            const int npar = prog->npars[s];
+            HIPCHK(hipMemcpyAsync(parp, b->start_params, sizeof(double) * 2 * npar, hipMemcpyHostToDevice, ctx->stream));
+            ka.start_params = parp;
+            parp += 2 * npar;
+            HIPCHK(hipMemcpyAsync(parp, b->target_params, sizeof(double) * 2 * npar * b->ngroups, hipMemcpyHostToDevice, ctx->stream));
+            ka.target_params = parp;
+            parp += 2L * npar * b->ngroups;
+        }
+        ka.next_path = ctx->d_next;
+        ka.opts = ctx->d_opts;
+        ka.results = (hcg_path_result*)d_res.p + off;
+        ka.solutions = (double*)d_sol.p + 2 * offx;
+        ka.last_points = (double*)d_last.p + 2 * offx;
+        ka.valuations = (double*)d_val.p + offx;
+        ka.counters = ctx->d_counters;
+        ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
+        off += np;
+        offx += np * n;
+    }
+    // group indices after all parameter blocks (int32, 8-byte aligned start)
+    int32_t* grp = (int32_t*)parp;
+    for (int s = 0; s < S; s++)
+        if (prog->kinds[s] == HCG_PARAMETER && batches[s].npaths > 0 && batches[s].group_of_path) {
+            HIPCHK(hipMemcpyAsync(grp, batches[s].group_of_path, sizeof(int32_t) * batches[s].npaths, hipMemcpyHostToDevice,
+                                  ctx->stream));
+            m.sys[s].group_of_path = grp;
+            grp += batches[s].npaths;
+        }
+    m.offset[S] = (int32_t)total;
+    const int block = 64;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
+    long grid = std::min<long>((total + block - 1) / block, (long)ctx->ncu * per_cu);
+    if (grid < 1) grid = 1;
+    void* args[] = {&m};
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->last_counters, ctx->d_counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(results, d_res.p, sizeof(hcg_path_result) * total, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(solutions, d_sol.p, sizeof(double) * 2 * tot_nx, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(last_points, d_last.p, sizeof(double) * 2 * tot_nx, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(valuations, d_val.p, sizeof(double) * tot_nx, hipMemcpyDeviceToHost, ctx->stream));
     return finish_launch(ctx);
 }
 

@@ -204,5 +204,6 @@ HCGPU_SYMBOLS = [
     "hcg_ctx_create", "hcg_ctx_destroy", "hcg_program_create", "hcg_program_destroy", "hcg_program_source",
     "hcg_default_options", "hcg_track_batch", "hcg_last_kernel_stats", "hcg_mlp_load",
     "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error", "hcg_compile_only", "hcg_generate",
-    "hcg_last_counters",
+    "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
+    "hcg_track_multi",
 ]

@@ -99,3 +99,54 @@ def cfg2_problem(ntargets: int = 256):
     p0 = parameter_draws(16, 1, seed=CFG2_SEEDS["p0"])[0]
     Q = parameter_draws(16, 256, seed=CFG2_SEEDS["targets"])[:ntargets]
     return F, p0, Q
+
+
+def shared_factor_exprs(n: int, seed: int = 0):
+    """The same system as shared_factor_system(n, seed), kept factored for evaluation:
+    f_i = l(x) q_i(x) + r_i(x) as an expression DAG (l shared by all equations)."""
+    from .expr import ExprSystem, expr_variables
+    rng = np.random.default_rng(seed)
+    xs, _ = expr_variables(n)
+    lc = cgauss(rng, n)
+    lin = xs[0] * complex(lc[0])
+    for c, x in zip(lc[1:], xs[1:]):
+        lin = lin + complex(c) * x
+    quad = monomials(n, 2, exact=True)
+    low = monomials(n, 2)
+
+    def mono(e):
+        m = None
+        for i, k in enumerate(e):
+            for _ in range(k):
+                m = xs[i] if m is None else m * xs[i]
+        return m
+
+    exprs = []
+    for i in range(n):
+        qc = cgauss(rng, len(quad))
+        rc = cgauss(rng, len(low))
+        q = None
+        for c, e in zip(qc, quad):
+            t = complex(c) * mono(e)
+            q = t if q is None else q + t
+        r = None
+        for c, e in zip(rc, low):
+            t = complex(c) if sum(e) == 0 else complex(c) * mono(e)
+            r = t if r is None else r + t
+        exprs.append(lin * q + r)
+    return ExprSystem(exprs, n)
+
+
+# The mixed-size, endgame-heavy workload: one system per size, all tracked in one launch.
+CFG4_SIZES = (4, 6, 8, 12)
+CFG4_SEEDS = dict(system=40, gamma=41)
+
+
+def cfg4_problem(sizes=CFG4_SIZES):
+    """[(H, starts)] per size: total-degree homotopies of shared-factor systems."""
+    from .homotopies import total_degree
+    out = []
+    for k, n in enumerate(sizes):
+        F = shared_factor_exprs(n, seed=CFG4_SEEDS["system"] + n)
+        out.append(total_degree(F, seed=CFG4_SEEDS["gamma"] + n))
+    return out

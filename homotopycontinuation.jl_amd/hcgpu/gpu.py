@@ -32,6 +32,14 @@ class GpuContext:
             self._programs[key] = p
         return p
 
+    def program_multi(self, homotopies) -> "MultiProgram":
+        key = tuple(id(h) for h in homotopies)
+        p = self._programs.get(key)
+        if p is None:
+            p = MultiProgram(self, list(homotopies))
+            self._programs[key] = p
+        return p
+
     def last_kernel_ms(self) -> float:
         ms = ctypes.c_double()
         check(lib().hcg_last_kernel_stats(self._h, ctypes.byref(ms), None, None))
@@ -165,3 +173,86 @@ def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, ta
     if log_arr is not None:
         log = log_arr[: min(keep[-2].value, log_capacity)]
     return BatchOutput(rec, sol, last, val, log)
+
+
+class MultiProgram:
+    """Several homotopies compiled into one program (hcg_program_create_multi): one launch
+    tracks the paths of all of them from a single work queue."""
+
+    def __init__(self, ctx: GpuContext, homotopies):
+        if not 1 <= len(homotopies) <= 8:
+            raise ValueError("between 1 and 8 homotopies per program")
+        self.ctx = ctx
+        self.homotopies = homotopies
+        self.ns = [h.n for h in homotopies]
+        self._h = ctypes.c_void_p()
+        descs = [h.desc() for h in homotopies]
+        arr = (ctypes.POINTER(abi.hcg_homotopy_desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
+        L = lib()
+        L.hcg_program_create_multi.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(abi.hcg_homotopy_desc)),
+                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        check(L.hcg_program_create_multi(ctx.handle, arr, len(descs), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().hcg_program_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+
+def track_multi(program: MultiProgram, starts_list, opts: Optional[hcg_options] = None, target_groups_list=None,
+                group_of_path_list=None):
+    """Track the start solutions of every system of `program` in one launch; returns one
+    BatchOutput per system."""
+    opts = opts or abi.default_options()
+    S = len(program.homotopies)
+    if len(starts_list) != S:
+        raise ValueError("one array of start solutions per system")
+    batches = (hcg_batch * S)()
+    keep = []
+    total = totx = 0
+    for s, (H, st) in enumerate(zip(program.homotopies, starts_list)):
+        st = np.ascontiguousarray(np.asarray(st, dtype=np.complex128))
+        if st.ndim != 2 or st.shape[1] != H.n:
+            raise ValueError(f"system {s}: start solutions must be npaths x {H.n}")
+        keep.append(st)
+        b = batches[s]
+        b.npaths = st.shape[0]
+        b.starts = _dp(st)
+        b.t1, b.t0 = 1.0, 0.0
+        if getattr(H, "q", None) is not None:
+            tg = None if target_groups_list is None else target_groups_list[s]
+            p = np.ascontiguousarray(H.p)
+            q = np.ascontiguousarray(H.q if tg is None else np.asarray(tg, dtype=np.complex128))
+            keep += [p, q]
+            b.start_params, b.target_params = _dp(p), _dp(q)
+            b.ngroups = q.shape[0]
+            gp = None if group_of_path_list is None else group_of_path_list[s]
+            if gp is not None:
+                g = np.ascontiguousarray(np.asarray(gp, dtype=np.int32))
+                if g.shape != (st.shape[0],):
+                    raise ValueError("group_of_path must have one entry per path")
+                keep.append(g)
+                b.group_of_path = g.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        total += st.shape[0]
+        totx += st.shape[0] * H.n
+    rec = np.zeros(total, dtype=_REC_DTYPE)
+    sol = np.zeros(totx, dtype=np.complex128)
+    last = np.zeros(totx, dtype=np.complex128)
+    val = np.zeros(totx, dtype=np.float64)
+    L = lib()
+    L.hcg_track_multi.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(hcg_batch),
+                                  ctypes.POINTER(hcg_options), ctypes.POINTER(hcg_path_result),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]
+    check(L.hcg_track_multi(program.ctx.handle, program._h, batches, ctypes.byref(opts),
+                            rec.ctypes.data_as(ctypes.POINTER(hcg_path_result)), _dp(sol), _dp(last), _dp(val)))
+    outs = []
+    off = offx = 0
+    for s, H in enumerate(program.homotopies):
+        np_ = batches[s].npaths
+        n = H.n
+        outs.append(BatchOutput(rec[off:off + np_], sol[offx:offx + np_ * n].reshape(np_, n),
+                                last[offx:offx + np_ * n].reshape(np_, n), val[offx:offx + np_ * n].reshape(np_, n)))
+        off += np_
+        offx += np_ * n
+    return outs

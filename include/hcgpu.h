@@ -200,6 +200,18 @@ int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batc
 
 /* Time of the last hcg_track_batch's tracker kernel (ms, HIP events on its stream),
  * and the device-side counters of that launch. */
+/* Several homotopies (1..8, any sizes and kinds) compiled into one program: one
+ * persistent launch tracks the paths of all of them from a single work queue.  This is the
+ * batch form of calling threaded_solve (src/solve.jl:628-709) once per system; there is
+ * no single reference call it replaces. */
+int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, hcg_program** out);
+int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, size_t* code_size);
+/* batches[s] describes the paths of system s.  Outputs are concatenated in system order:
+ * results: sum_s npaths_s records; solutions and last_points: sum_s npaths_s * n_s complex;
+ * valuations: sum_s npaths_s * n_s doubles. */
+int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches, const hcg_options* opts,
+                    hcg_path_result* results, double* solutions, double* last_points, double* valuations);
+
 int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps,
                           int64_t* total_newton_iters);
 /* Device counters of the last launch: [0] tracker steps, [1] Newton iterations,
