@@ -1,16 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: paths/s of the MI355X batched path tracker on one GPU.
 
-Workload (one "step"): the dense total-degree solve -- n = 8 variables, 8 dense cubic
-polynomials with all 165 monomials of degree <= 3, iid complex Gaussian coefficients
-(seed 1), total-degree start system x_i^3 - 1, 6,561 paths, straight-line homotopy with
-γ from seed 2, ComplexF64 -- tracked to t = 0 with the endgame in ONE persistent
-kernel launch.  Inputs are resident in HBM before the timed region (torch allocates
-the buffers, libhcgpu's hcg_track_batch_device launches on them).
+Default workload ("param10", one step = one launch over the whole batch): n = 10 sparse
+quadrics (20 monomials each) whose coefficients are affine in 16 complex parameters;
+1,024 start solutions at p0 (tests/golden/cfg2_starts.npy) tracked to 256 target
+parameter vectors = 262,144 paths of uneven length in ONE persistent launch, ComplexF64,
+with the endgame.  Timed: the learned-step tracker (`value`) and the plain tracker, both
+with inputs resident in HBM (hcg_track_batch_device).  Beside them: the CPU oracle (a C
+restatement of the reference tracker) on a bounded sample of the same batch.
 
-The JSON line carries the counted-FLOP roofline of the tracker kernel (HIP events on
-the launch stream) and the CPU baseline: the oracle (a C restatement of the
-reference tracker) on 16 host threads over the same 6,561 paths.
+Other workloads (--workload): "dense8" = 8 dense cubics in 8 unknowns, 6,561 total-degree
+paths; "mixed" = shared-factor systems of n = 4, 6, 8, 12 (538,812 paths) in one launch.
+
+The JSON line carries the counted-FLOP roofline of the timed kernel (HIP events on the
+launch stream) and, when profiled, its HBM traffic from profiles/traffic.json.
 """
 import argparse
 import ctypes
@@ -24,7 +27,7 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "homotopycontinuation.jl_amd"))
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
-HBM_PEAK_GBS = 8000.0
+METRIC = "paths/sec: learned-step GPU vs plain GPU vs host CPU, same solution sets"
 
 
 def measured_traffic(workload: str):
@@ -37,21 +40,249 @@ def measured_traffic(workload: str):
     return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0)
 
 
-def build_workload(learned: bool):
+def roofline(flops, kernel_ms, workload):
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "bound_detail": "FP64 compute roof (MI355X FP64 vector = matrix peak 78.6 TFLOP/s)",
+            "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": measured_traffic(workload),
+            "kernel": "hcg_track_kernel", "kernel_ms": round(kernel_ms, 3), "flops_per_launch": flops}
+
+
+class DeviceBatch:
+    """A batch already resident in HBM (torch allocations), launched with hcg_track_batch_device."""
+
+    def __init__(self, ctx, prog, starts, p=None, Q=None, groups=None):
+        import numpy as np
+        import torch
+        from hcgpu.abi import hcg_batch, hcg_path_result
+        dev = torch.device("cuda:0")
+        self.npaths, self.n = starts.shape
+        self.ctx, self.prog = ctx, prog
+
+        def t(a):
+            return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+        self.starts = t(np.ascontiguousarray(starts).view(np.float64))
+        self.p = t(None if p is None else np.ascontiguousarray(p).view(np.float64))
+        self.q = t(None if Q is None else np.ascontiguousarray(Q).view(np.float64))
+        self.g = t(groups)
+        self.res = torch.empty(self.npaths * ctypes.sizeof(hcg_path_result), dtype=torch.uint8, device=dev)
+        self.sol = torch.empty(self.npaths * self.n * 2, dtype=torch.float64, device=dev)
+        self.last = torch.empty_like(self.sol)
+        self.val = torch.empty(self.npaths * self.n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        b = hcg_batch()
+        b.npaths = self.npaths
+        P = ctypes.POINTER(ctypes.c_double)
+        b.starts = ctypes.cast(self.starts.data_ptr(), P)
+        b.t1, b.t0 = 1.0, 0.0
+        if self.p is not None:
+            b.start_params = ctypes.cast(self.p.data_ptr(), P)
+            b.target_params = ctypes.cast(self.q.data_ptr(), P)
+            b.ngroups = Q.shape[0]
+            b.group_of_path = ctypes.cast(self.g.data_ptr(), ctypes.POINTER(ctypes.c_int32))
+        self.b = b
+
+    def launch(self, opts):
+        from hcgpu import abi
+        L = abi.lib()
+        L.hcg_track_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(abi.hcg_batch),
+                                             ctypes.POINTER(abi.hcg_options), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        abi.check(L.hcg_track_batch_device(self.ctx.handle, self.prog._h, ctypes.byref(self.b), ctypes.byref(opts),
+                                           self.res.data_ptr(), self.sol.data_ptr(), self.last.data_ptr(),
+                                           self.val.data_ptr()))
+        return self.ctx.last_kernel_ms(), self.ctx.last_counters()
+
+    def results(self):
+        import numpy as np
+        from hcgpu.abi import hcg_path_result
+        dt = np.dtype([(f, np.int32 if t is ctypes.c_int32 else np.float64) for f, t in hcg_path_result._fields_],
+                      align=True)
+        rec = np.frombuffer(self.res.cpu().numpy().tobytes(), dtype=dt)
+        sol = self.sol.cpu().numpy().view(np.complex128).reshape(self.npaths, self.n)
+        return rec, sol
+
+
+def timed(batch, opts, steps, warmup):
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        batch.launch(opts)
+    torch.cuda.synchronize()
+    kms, cs = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms, c = batch.launch(opts)
+        kms.append(ms)
+        cs.append(c)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(wall=wall, kernel_ms=float(np.mean(kms)), flops=float(np.mean([c["flops"] for c in cs])),
+                steps=float(np.mean([c["steps"] for c in cs])), newton=float(np.mean([c["newton_iters"] for c in cs])))
+
+
+def counts(rec):
+    import numpy as np
+    code, sing = rec["code"], rec["singular"] != 0
+    return dict(success=int((code == 1).sum()), nonsingular=int(((code == 1) & ~sing).sum()),
+                singular=int(((code == 1) & sing).sum()), at_infinity=int(np.isin(code, (2, 3)).sum()),
+                failed=int((~np.isin(code, (1, 2, 3))).sum()))
+
+
+def learned_weights(ctx, F, p0, S):
+    """The committed network for this workload, or one trained now on the GPU from the plain
+    tracker's step log over 16 training targets (seeded draws disjoint from the benchmark's)."""
+    import numpy as np
+    from hcgpu import learned
+    from hcgpu.configs import parameter_draws
+    from hcgpu.gpu import track_batch
+    from hcgpu.homotopies import ParameterHomotopy
+    from hcgpu.solve import make_options
+    w = learned.saved_weights("param10_step_mlp")
+    if w is not None:
+        return w, "committed (hcgpu/weights/param10_step_mlp.npy)"
+    Qt = parameter_draws(16, 16, seed=30)
+    Ht = ParameterHomotopy(F, p0, Qt, lift_affine=True)
+    st = np.tile(S, (len(Qt), 1))
+    gt = np.repeat(np.arange(len(Qt), dtype=np.int32), len(S))
+    out = track_batch(ctx.program(Ht), st, make_options(), group_of_path=gt, log_capacity=2_000_000)
+    w, loss, info = learned.fit_from_log(ctx, out.log, out.code)
+    return w, f"trained now on {info['records']} step records (loss {loss:.4f})"
+
+
+def bench_param10(args, out):
+    import numpy as np
+    from hcgpu import learned
+    from hcgpu.configs import cfg2_problem
+    from hcgpu.gpu import GpuContext
+    from hcgpu.homotopies import ParameterHomotopy
+    from hcgpu.solve import make_options
+    F, p0, Q = cfg2_problem(256)
+    S = np.load(REPO / "tests" / "golden" / "cfg2_starts.npy")
+    H = ParameterHomotopy(F, p0, Q, lift_affine=True)
+    starts = np.tile(S, (len(Q), 1))
+    groups = np.repeat(np.arange(len(Q), dtype=np.int32), len(S))
+    ctx = GpuContext(0)
+    prog = ctx.program(H)
+    batch = DeviceBatch(ctx, prog, starts, H.p, H.q, groups)
+    npaths = len(starts)
+
+    w, wsrc = learned_weights(ctx, F, p0, S)
+    learned.load(ctx, w)
+    plain_o, learned_o = make_options(), make_options(learned_step=1)
+    rp = timed(batch, plain_o, args.steps, args.warmup)
+    rec_p, sol_p = batch.results()
+    rl = timed(batch, learned_o, args.steps, args.warmup)
+    rec_l, sol_l = batch.results()
+
+    value = npaths * args.steps / rl["wall"]
+    plain_value = npaths * args.steps / rp["wall"]
+    ok = (rec_p["code"] == 1) & (rec_l["code"] == 1) & (rec_p["singular"] == 0)
+    rel = (np.abs(sol_l[ok] - sol_p[ok]).max(axis=1) / np.maximum(1.0, np.abs(sol_p[ok]).max(axis=1))) if ok.any() \
+        else np.zeros(1)
+    lp = int(rec_l["learned_proposed"].sum())
+    out.update({
+        "value": round(value, 1), "ms_per_step": round(rl["wall"] / args.steps * 1e3, 3),
+        "config": {"workload": "param10: n=10 sparse quadrics (20 monomials, coefficients affine in 16 params), "
+                               "1024 starts x 256 targets = 262144 paths, one persistent launch, learned-step tracker",
+                   "n": 10, "paths": npaths, "starts": len(S), "targets": len(Q), "params": 16,
+                   "affine_parameter_lift": True, "weights": wsrc},
+        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned"),
+        "plain_gpu": {"value": round(plain_value, 1), "unit": "paths/s", "kernel_ms": round(rp["kernel_ms"], 3),
+                      "steps_per_path": round(rp["steps"] / npaths, 3),
+                      "roofline_frac": round(rp["flops"] / (rp["kernel_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)},
+        "learned": {"steps_per_path": round(rl["steps"] / npaths, 3), "proposals": lp,
+                    "acceptance_rate": round(int(rec_l["learned_accepted"].sum()) / max(1, lp), 4),
+                    "speedup_vs_plain": round(value / plain_value, 3)},
+        "steps_per_path": round(rl["steps"] / npaths, 3),
+        "newton_iters_per_step": round(rl["newton"] / max(1.0, rl["steps"]), 3),
+        "solution_counts": counts(rec_l),
+        "parity_learned_vs_plain": {"identical_return_codes": bool((rec_l["code"] == rec_p["code"]).all()),
+                                    "max_rel_endpoint_diff": float(rel.max())},
+    })
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        import oracle  # CPU baseline only (never the measured GPU path)
+        m = args.cpu_targets * len(S)
+        t0 = time.perf_counter()
+        o = oracle.track_batch(H, starts[:m], plain_o, nthreads=args.cpu_threads, group_of_path=groups[:m])
+        dt = time.perf_counter() - t0
+        okc = (o.code == 1) & (~o.singular)
+        relc = np.abs(sol_l[:m][okc] - o.solution[okc]).max(axis=1) / np.maximum(1.0, np.abs(o.solution[okc]).max(axis=1))
+        out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "paths/s", "cores": args.cpu_threads, "kind": "port",
+                               "sample": f"first {args.cpu_targets} of the 256 targets ({m} paths) of the same batch, "
+                                         f"oracle (C restatement of the reference tracker, plain rule), {dt:.1f} s"}
+        out["gpu_vs_cpu"] = round(value / (m / dt), 2)
+        out["plain_gpu_vs_cpu"] = round(plain_value / (m / dt), 2)
+        out["parity_vs_oracle_sample"] = {
+            "identical_return_codes": bool((o.code == rec_l["code"][:m]).all()),
+            "max_rel_endpoint_diff_learned": float(relc.max()) if okc.any() else 0.0}
+
+
+def bench_dense8(args, out):
     from hcgpu.configs import dense_system
+    from hcgpu.gpu import GpuContext
     from hcgpu.homotopies import total_degree
-    F = dense_system(8, 3, seed=1)
-    H, S = total_degree(F, seed=2)
-    return H, S
+    from hcgpu.solve import make_options
+    H, S = total_degree(dense_system(8, 3, seed=1), seed=2)
+    ctx = GpuContext(0)
+    batch = DeviceBatch(ctx, ctx.program(H), S)
+    r = timed(batch, make_options(), args.steps, args.warmup)
+    rec, _ = batch.results()
+    npaths = len(S)
+    out.update({"value": round(npaths * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+                "config": {"workload": "dense8: n=8, 8 dense cubics (165 monomials each), 6561 total-degree paths, "
+                                       "one persistent launch, plain tracker", "n": 8, "paths": npaths},
+                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths"),
+                "steps_per_path": round(r["steps"] / npaths, 3), "solution_counts": counts(rec)})
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        import oracle  # CPU baseline only
+        t0 = time.perf_counter()
+        o = oracle.track_batch(H, S, make_options(), nthreads=args.cpu_threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(npaths / dt, 1), "unit": "paths/s", "cores": args.cpu_threads,
+                               "kind": "port", "sample": f"all {npaths} paths, oracle, {dt:.1f} s"}
+        out["parity_vs_oracle"] = {"identical_return_codes": bool((o.code == rec["code"]).all())}
+
+
+def bench_mixed(args, out):
+    import numpy as np
+    from hcgpu.configs import cfg4_problem
+    from hcgpu.gpu import GpuContext, track_multi
+    from hcgpu.solve import make_options
+    probs = cfg4_problem()
+    ctx = GpuContext(0)
+    prog = ctx.program_multi([H for H, _ in probs])
+    starts = [S for _, S in probs]
+    total = sum(len(S) for S in starts)
+    for _ in range(args.warmup):
+        track_multi(prog, starts, make_options())
+    kms, fl = [], []
+    outs = None
+    for _ in range(args.steps):
+        outs = track_multi(prog, starts, make_options())
+        kms.append(ctx.last_kernel_ms())
+        fl.append(ctx.last_counters()["flops"])
+    ms = float(np.mean(kms))
+    out.update({"value": round(total / ms * 1e3, 1), "ms_per_step": round(ms, 3),
+                "config": {"workload": "mixed: shared-factor systems n=4,6,8,12 (538812 total-degree paths) in one "
+                                       "launch; value from the kernel time (inputs copied from the host each launch)",
+                           "paths": total},
+                "roofline": roofline(float(np.mean(fl)), ms, "mixed"),
+                "solution_counts": {f"n{H.n}": counts(o.records) for (H, _), o in zip(probs, outs)}})
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-targets", type=int, default=32, help="targets of the CPU baseline sample (param10)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one plain device-to-device copy of this many MiB (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counter readings) and exit")
@@ -60,10 +291,7 @@ def main():
         print("bench.py measures one GPU only: paths are independent and this build has no multi-GPU path "
               "(see DESIGN.md for how the batch would shard)", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
     import torch
-
     if args.copy_calibration_mib:
         nbytes = args.copy_calibration_mib << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
@@ -73,110 +301,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "read_bytes": nbytes, "written_bytes": nbytes}))
         return
-
-    from hcgpu import abi
-    from hcgpu.abi import check, hcg_batch, hcg_path_result, lib
-    from hcgpu.gpu import GpuContext
-    from hcgpu.solve import make_options
-
-    H, S = build_workload(False)
-    npaths, n = S.shape
-    ctx = GpuContext(0)
-    prog = ctx.program(H)  # hipRTC (code-object cache)
-    opts = make_options()
-
-    dev = torch.device("cuda:0")
-    starts = torch.from_numpy(np.ascontiguousarray(S).view(np.float64).copy()).to(dev)
-    res = torch.empty(npaths * ctypes.sizeof(hcg_path_result), dtype=torch.uint8, device=dev)
-    sol = torch.empty(npaths * n * 2, dtype=torch.float64, device=dev)
-    last = torch.empty_like(sol)
-    val = torch.empty(npaths * n, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    b = hcg_batch()
-    b.npaths = npaths
-    b.starts = ctypes.cast(starts.data_ptr(), ctypes.POINTER(ctypes.c_double))
-    b.t1, b.t0 = 1.0, 0.0
-    L = lib()
-    L.hcg_track_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(hcg_batch),
-                                         ctypes.POINTER(abi.hcg_options), ctypes.c_void_p, ctypes.c_void_p,
-                                         ctypes.c_void_p, ctypes.c_void_p]
-
-    def launch():
-        check(L.hcg_track_batch_device(ctx.handle, prog._h, ctypes.byref(b), ctypes.byref(opts), res.data_ptr(),
-                                       sol.data_ptr(), last.data_ptr(), val.data_ptr()))
-        return ctx.last_kernel_ms(), ctx.last_counters()
-
-    for _ in range(args.warmup):
-        launch()
-    torch.cuda.synchronize()
-    kms, flops, steps, newton = [], [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ms, c = launch()
-        kms.append(ms)
-        flops.append(c["flops"])
-        steps.append(c["steps"])
-        newton.append(c["newton_iters"])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-
-    # results of the last launch
-    rec = np.frombuffer(res.cpu().numpy().tobytes(), dtype=np.dtype(
-        [(f, np.int32 if t is ctypes.c_int32 else np.float64) for f, t in hcg_path_result._fields_], align=True))
-    gsol = sol.cpu().numpy().view(np.complex128).reshape(npaths, n)
-    nonsing = int(((rec["code"] == 1) & (rec["singular"] == 0)).sum())
-    counts = dict(success=int((rec["code"] == 1).sum()), nonsingular=nonsing,
-                  singular=int(((rec["code"] == 1) & (rec["singular"] != 0)).sum()),
-                  at_infinity=int(((rec["code"] == 2) | (rec["code"] == 3)).sum()),
-                  failed=int(((rec["code"] != 1) & (rec["code"] != 2) & (rec["code"] != 3)).sum()))
-
-    value = npaths * args.steps / wall
-    mean_ms = float(np.mean(kms))
-    mean_flops = float(np.mean(flops))
-    achieved = mean_flops / (mean_ms * 1e-3) / 1e12
-    out = {
-        "metric": "paths/sec: learned-step GPU vs plain GPU vs host CPU, same solution sets",
-        "value": round(value, 1),
-        "unit": "paths/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded complex Gaussian coefficients)",
-        "config": {"workload": "dense total-degree solve: n=8, 8 dense cubics (165 monomials each), "
-                               "6561 paths, one persistent launch (configs[1])",
-                   "n": n, "paths": npaths, "system_seed": 1, "gamma_seed": 2, "tracker": "plain (adaptive rule)"},
-        "roofline": {"bound": "mfma", "bound_detail": "FP64 compute roof (MI355X FP64 vector = matrix peak)",
-                     "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": measured_traffic("dense_n8_6561paths"),
-                     "kernel": "hcg_track_kernel", "kernel_ms": round(mean_ms, 3),
-                     "flops_per_launch": mean_flops, "flops_per_step": mean_flops / max(1, np.mean(steps))},
-        "steps_per_path": round(float(np.mean(steps)) / npaths, 3),
-        "newton_iters_per_step": round(float(np.mean(newton)) / max(1.0, float(np.mean(steps))), 3),
-        "solution_counts": counts,
-    }
-
-    if not args.no_cpu:
-        sys.path.insert(0, str(REPO))
-        import oracle  # CPU baseline only (checker / baseline, never the measured GPU path)
-        threads = args.cpu_threads
-        t0 = time.perf_counter()
-        o = oracle.track_batch(H, S, opts, nthreads=threads)
-        dt = time.perf_counter() - t0
-        same_codes = bool((o.code == rec["code"]).all())
-        ok = (o.code == 1) & (~o.singular)
-        rel = np.abs(gsol[ok] - o.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(o.solution[ok]).max(axis=1))
-        out["cpu_baseline"] = {"value": round(npaths / dt, 1), "unit": "paths/s", "cores": threads, "kind": "port",
-                               "sample": f"all {npaths} paths of the workload, oracle (C restatement of the "
-                                         f"reference tracker) on {threads} threads"}
-        out["gpu_vs_cpu"] = round(value / (npaths / dt), 2)
-        out["parity"] = {"identical_return_codes": same_codes,
-                         "max_rel_endpoint_diff": float(rel.max()) if ok.any() else 0.0}
+    out = {"metric": METRIC, "value": None, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}
+    {"param10": bench_param10, "dense8": bench_dense8, "mixed": bench_mixed}[args.workload](args, out)
     print(json.dumps(out))
 
 
