@@ -587,7 +587,7 @@ bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::st
         }
         return out + "            }\n";
     };
-    os << R"(extern "C" __global__ void __launch_bounds__(64) hcg_track_multi_kernel(hcg_multi_kargs m) {
+    os << R"(extern "C" __global__ void __launch_bounds__(64, HCG_WAVES_PER_EU) hcg_track_multi_kernel(hcg_multi_kargs m) {
     __shared__ unsigned short mlp_lds[HCG_MLP_NPARAMS];
     const unsigned short* mw = stage_mlp(m.sys[0].mlp, m.sys[0].opts, mlp_lds);
     hcg_lane_union U;

@@ -3,6 +3,11 @@
 #pragma once
 #include "hcgpu.h"
 
+// Waves per SIMD the tracker kernels are compiled for (register budget 512 / this).
+#ifndef HCG_WAVES_PER_EU
+#define HCG_WAVES_PER_EU 1
+#endif
+
 typedef struct hcg_kargs {
     int32_t npaths;
     int32_t log_capacity;

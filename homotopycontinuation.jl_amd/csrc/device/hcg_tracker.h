@@ -18,7 +18,7 @@
 #include "hcg_mlp_infer.h"
 #include "hcg_tracker_body.h"
 
-extern "C" __global__ void __launch_bounds__(64) hcg_track_kernel(hcg_kargs a) {
+extern "C" __global__ void __launch_bounds__(64, HCG_WAVES_PER_EU) hcg_track_kernel(hcg_kargs a) {
     __shared__ unsigned short mlp_lds[HCG_MLP_NPARAMS];
     const unsigned short* mw = stage_mlp(a.mlp, a.opts, mlp_lds);
     Lane L;

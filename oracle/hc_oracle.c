@@ -922,7 +922,7 @@ typedef struct {
     cdd* xdd;
     /* Predictor, src/predictor.jl:72-103 */
     int pred_hermite; int pred_order;
-    double trust_region, local_error, cond_H_xd;
+    double trust_region, local_error, cond_H_xd, ds_plain;
     cplx *tx0, *tx1, *tx2, *tx3; /* x, x¹, x², x³ */
     cplx pred_t, pred_prev_t;
     double tx_norm[4];
@@ -1385,6 +1385,7 @@ static void update_stepsize(orc_tracker* T, const newton_result* res) {
         if (T->use_strict_beta_tau && seg_dist_to_target(T) < ds) ds *= o->strict_beta_tau;
         ds = jmin(ds, 10 * T->ds_prev);
         if (T->last_steps_failed > 0) ds = jmin(ds, T->ds_prev);
+        T->ds_plain = ds;
     } else {
         int j = res->iters - 2;
         double Th = (j >= 0) ? nthroot(res->theta, 1 << j) : 1.0;
@@ -1532,7 +1533,8 @@ static void step_features(orc_tracker* T, const newton_result* res, double ds0, 
 
 /* step record: 0 path, 1 step index, 2-9 features, 10 accepted, 11 Newton iterations,
  * 12 log of the next proposed Δs, 13 flags (1 extended precision, 2 singular endgame),
- * 14 log10 accuracy, 15 log of the plain rule's next proposal (same as 12 here) */
+ * 14 log10 accuracy, 15 log of the plain rule's next proposal before the segment end clamps it
+ * (after a rejected step: same as 12) */
 static void log_step(orc_tracker* T, const float* feat, const newton_result* res, int acc) {
     int idx = atomic_fetch_add(T->log_count, 1);
     if (idx >= T->log_capacity) return;
@@ -1545,7 +1547,7 @@ static void log_step(orc_tracker* T, const float* feat, const newton_result* res
     r[12] = (float)hc_log(seg_ds(T));
     r[13] = (float)(T->extended_prec + 2 * T->singular_endgame);
     r[14] = (float)(hc_log(T->accuracy) / 2.302585092994046);
-    r[15] = r[12];
+    r[15] = isnan(T->ds_plain) ? r[12] : (float)hc_log(T->ds_plain);
 }
 
 /* step!(tracker), src/tracker.jl:851-926; returns !last_step_failed */
@@ -1583,6 +1585,7 @@ static int tracker_step(orc_tracker* T) {
     T->norm_dx0 = res.norm_dx0;
     float feat[8];
     if (T->log_records) step_features(T, &res, ds0, feat);
+    T->ds_plain = NAN;
     update_stepsize(T, &res);
     if (T->log_records) log_step(T, feat, &res, res.code == NEWT_CONVERGED);
     check_terminated(T);
