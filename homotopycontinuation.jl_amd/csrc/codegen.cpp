@@ -83,7 +83,7 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
                      ParamMode pm, const double* fixed, long* flops) {
     const char* T = dd ? "cdd" : "cplx";
     os << "__device__ __noinline__ void " << fname << "(const " << T << "* __restrict__ x, ";
-    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, ";
+    if (pm == PM_DYNAMIC) os << "const double* __restrict__ pp, const double* __restrict__ pq, cplx pt, ";
     os << (dd ? "cdd" : "cplx") << "* __restrict__ u";
     if (jac) os << ", cplx* __restrict__ U";
     os << ") {\n";
@@ -91,24 +91,32 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     auto wrap = [&](const std::string& c) { return dd ? "cdd_from(" + c + ")" : c; };
     for (int k = 0; k < tp->nconst; k++)
         nm[k + 1] = wrap(clit(tp->constants[2 * k], tp->constants[2 * k + 1]));
+    // parameters and variables are defined right before their first use, so that their live
+    // ranges (and the register pressure of long straight-line bodies) stay short
+    std::vector<std::string> lazy_def(tp->tape_space + 1);
     for (int k = 0; k < tp->nparams; k++) {
         std::string v = "P" + std::to_string(k);
-        if (pm == PM_FIXED) os << "  const " << T << " " << v << " = " << wrap(clit(fixed[2 * k], fixed[2 * k + 1])) << ";\n";
-        else os << "  const " << T << " " << v << " = " << wrap("p[" + std::to_string(k) + "]") << ";\n";
+        if (pm == PM_FIXED) lazy_def[tp->params_first + k] = "  const " + std::string(T) + " " + v + " = " + wrap(clit(fixed[2 * k], fixed[2 * k + 1])) + ";\n";
+        else lazy_def[tp->params_first + k] = "  const " + std::string(T) + " " + v + " = " + wrap("par_lerp(pt, pp, pq, " + std::to_string(k) + ")") + ";\n";
         nm[tp->params_first + k] = v;
     }
     for (int k = 0; k < tp->nvars; k++) {
         std::string v = "X" + std::to_string(k);
-        os << "  const " << T << " " << v << " = x[" << k << "];\n";
+        lazy_def[tp->vars_first + k] = "  const " + std::string(T) + " " + v + " = x[" + std::to_string(k) + "];\n";
         nm[tp->vars_first + k] = v;
     }
+    auto need = [&](int slot) {
+        if (slot >= 1 && slot <= tp->tape_space && !lazy_def[slot].empty()) { os << lazy_def[slot]; lazy_def[slot].clear(); }
+    };
     const char* pre = dd ? "cdd_" : "c_";
-    long f = 0;
+    long f = pm == PM_DYNAMIC ? 6L * tp->nparams : 0;  // p(t) = t p + (1 - t) q per parameter
     long dm = dd ? 58 : 6, da = dd ? 22 : 2;
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
         if (op == OP_STOP) break;
+        for (int q = 0; q < arity(op); q++)
+            if (!(op == OP_POW_INT && q == 1)) need(I[q]);
         std::string a = nm[I[0]], b = arity(op) > 1 && op != OP_POW_INT ? nm[I[1]] : "", c = arity(op) > 2 ? nm[I[2]] : "",
                     d = arity(op) > 3 ? nm[I[3]] : "";
         std::string out = "T" + std::to_string(i);
@@ -142,6 +150,7 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     std::vector<int> uset(n, 0);
     for (int i = 0; i < tp->nu; i++) {
         int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
+        need(slot);
         os << "  u[" << row << "] = " << nm[slot] << ";\n";
         uset[row] = 1;
     }
@@ -151,6 +160,7 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
         std::vector<int> Uset(n * n, 0);
         for (int i = 0; i < tp->nU; i++) {
             int l = tp->U_assign[2 * i] - 1, slot = tp->U_assign[2 * i + 1];
+            need(slot);
             os << "  U[" << l << "] = " << nm[slot] << ";\n";
             Uset[l] = 1;
         }
@@ -222,8 +232,21 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
     std::vector<int> free_regs;
     int nreg = 0;
     auto alloc = [&]() { if (!free_regs.empty()) { int r = free_regs.back(); free_regs.pop_back(); return r; } return nreg++; };
-    std::vector<std::pair<int, int>> loads;  // (register, input index) for params+vars
-    for (int k = 0; k < tp->nparams + tp->nvars; k++) {
+    // parameters are not copied into registers: fixed ones become constants, homotopy-driven
+    // ones are fetched as t p + (1 - t) q when an instruction reads them (operand DD_PBASE + k)
+    const int DD_PBASE = 1 << 24;
+    std::vector<int> par_operand(tp->nparams, 0);
+    for (int k = 0; k < tp->nparams; k++) {
+        if (pm == PM_FIXED) {
+            consts.push_back(fixed[2 * k]);
+            consts.push_back(fixed[2 * k + 1]);
+            par_operand[k] = -(int)(consts.size() / 2) ;
+        } else {
+            par_operand[k] = DD_PBASE + k;
+        }
+    }
+    std::vector<std::pair<int, int>> loads;  // (register, input index) for vars
+    for (int k = tp->nparams; k < tp->nparams + tp->nvars; k++) {
         if (def_last[k] < 0) continue;  // unused input
         reg[k] = alloc();
         loads.push_back({reg[k], k});
@@ -237,12 +260,13 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
             int d = in_defs[i][q];
             if (d == -2) ops[q] = (op == OP_POW_INT && q == 1) ? I[1] : 0;
             else if (d == -1) ops[q] = -(I[q] - 1) - 1;
+            else if (d < tp->nparams) ops[q] = par_operand[d];
             else ops[q] = reg[d];
         }
         // free inputs whose last use is this instruction
         for (int q = 0; q < 4; q++) {
             int d = in_defs[i][q];
-            if (d >= 0 && def_last[d] == i && reg[d] >= 0) {
+            if (d >= tp->nparams && def_last[d] == i && reg[d] >= 0) {
                 bool dup = false;
                 for (int r = 0; r < q; r++) if (in_defs[i][r] == d) dup = true;
                 if (!dup) free_regs.push_back(reg[d]);
@@ -255,7 +279,7 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
         code.insert(code.end(), {op, ops[0], ops[1], ops[2], ops[3], outr});
     }
     for (int r = 0; r < n; r++) {
-        if (out_is_def[r] >= 0) out_operand[r] = reg[out_is_def[r]];
+        if (out_is_def[r] >= 0) out_operand[r] = out_is_def[r] < tp->nparams ? par_operand[out_is_def[r]] : reg[out_is_def[r]];
         else if (out_const[r] >= 0) out_operand[r] = -out_const[r] - 1;
     }
     // fixed parameters become constants: rewrite their register loads
@@ -267,26 +291,19 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
     if (consts.empty()) os << "0.0";
     os << "};\n";
     os << "__device__ __noinline__ void " << fname << "(const cdd* __restrict__ x, ";
-    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, ";
+    if (pm == PM_DYNAMIC) os << "const double* __restrict__ pp, const double* __restrict__ pq, cplx pt, ";
     os << "cdd* __restrict__ u) {\n";
     os << "  cdd R[" << std::max(nreg, 1) << "];\n";
-    for (auto& ld : loads) {
-        int k = ld.second;
-        if (k < tp->nparams) {
-            if (pm == PM_FIXED) os << "  R[" << ld.first << "] = cdd_from(" << clit(fixed[2 * k], fixed[2 * k + 1]) << ");\n";
-            else os << "  R[" << ld.first << "] = cdd_from(p[" << k << "]);\n";
-        } else {
-            os << "  R[" << ld.first << "] = x[" << (k - tp->nparams) << "];\n";
-        }
-    }
-    os << "  dd_interpret(" << fname << "_code, " << ninstr << ", " << fname << "_const, R);\n";
+    os << "  DDPar P; P.p = " << (pm == PM_DYNAMIC ? "pp" : "nullptr") << "; P.q = " << (pm == PM_DYNAMIC ? "pq" : "nullptr")
+       << "; P.t = " << (pm == PM_DYNAMIC ? "pt" : "mkc(0.0, 0.0)") << ";\n";
+    for (auto& ld : loads) os << "  R[" << ld.first << "] = x[" << (ld.second - tp->nparams) << "];\n";
+    os << "  dd_interpret(" << fname << "_code, " << ninstr << ", " << fname << "_const, R, P);\n";
     for (int r = 0; r < n; r++) {
         if (out_operand[r] == INT32_MIN) os << "  u[" << r << "] = cdd_from(mkc(0.0, 0.0));\n";
-        else if (out_operand[r] >= 0) os << "  u[" << r << "] = R[" << out_operand[r] << "];\n";
-        else { int k = -out_operand[r] - 1; os << "  u[" << r << "] = cdd_from(" << clit(tp->constants[2 * k], tp->constants[2 * k + 1]) << ");\n"; }
+        else os << "  u[" << r << "] = dd_fetch(" << fname << "_const, R, P, " << out_operand[r] << ");\n";
     }
     os << "}\n";
-    long f = 0;
+    long f = pm == PM_DYNAMIC ? 6L * tp->nparams : 0;  // p(t) of every parameter
     for (int i = 0; i < ninstr; i++) {
         int op = code[6 * i];
         switch (op) {
@@ -310,7 +327,7 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
                        const double* fixed, long* flops) {
     os << "__device__ __noinline__ void " << fname << "(";
     for (int j = 0; j < K; j++) os << "const cplx* __restrict__ t" << j << ", ";
-    if (pm == PM_DYNAMIC) os << "const cplx* __restrict__ p, const cplx* __restrict__ dp, ";
+    if (pm == PM_DYNAMIC) os << "const double* __restrict__ pp, const double* __restrict__ pq, cplx pt, ";
     os << "cplx* __restrict__ oKm1, cplx* __restrict__ oK) {\n";
     std::vector<SVal> v(tp->tape_space + 1);
     auto zero = [&]() { SVal s; s.deg = -1; return s; };
@@ -319,16 +336,19 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         SVal s; s.deg = 0; s.c.push_back(clit(tp->constants[2 * k], tp->constants[2 * k + 1]));
         v[k + 1] = s;
     }
+    // parameters and variables are defined right before their first use (short live ranges)
+    std::vector<std::string> lazy_def(tp->tape_space + 1);
     for (int k = 0; k < tp->nparams; k++) {
         SVal s;
         std::string nm0 = "P" + std::to_string(k);
+        std::string& d = lazy_def[tp->params_first + k];
         if (pm == PM_FIXED) {
-            os << "  const cplx " << nm0 << " = " << clit(fixed[2 * k], fixed[2 * k + 1]) << ";\n";
+            d = "  const cplx " + nm0 + " = " + clit(fixed[2 * k], fixed[2 * k + 1]) + ";\n";
             s.deg = 0; s.c.push_back(nm0);
         } else {
-            os << "  const cplx " << nm0 << " = p[" << k << "];\n";
+            d = "  const cplx " + nm0 + " = par_lerp(pt, pp, pq, " + std::to_string(k) + ");\n";
             std::string nm1 = "Q" + std::to_string(k);
-            os << "  const cplx " << nm1 << " = dp[" << k << "];\n";
+            if (K >= 1) d += "  const cplx " + nm1 + " = par_diff(pp, pq, " + std::to_string(k) + ");\n";
             s.deg = K >= 1 ? 1 : 0; s.c.push_back(nm0);
             if (K >= 1) s.c.push_back(nm1);
         }
@@ -337,14 +357,18 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
     for (int k = 0; k < tp->nvars; k++) {
         SVal s;
         s.deg = K - 1;
+        std::string& d = lazy_def[tp->vars_first + k];
         for (int j = 0; j < K; j++) {
             std::string nm = "X" + std::to_string(k) + "_" + std::to_string(j);
-            os << "  const cplx " << nm << " = t" << j << "[" << k << "];\n";
+            d += "  const cplx " + nm + " = t" + std::to_string(j) + "[" + std::to_string(k) + "];\n";
             s.c.push_back(nm);
         }
         v[tp->vars_first + k] = s;
     }
-    long f = 0;
+    auto need = [&](int slot) {
+        if (slot >= 1 && slot <= tp->tape_space && !lazy_def[slot].empty()) { os << lazy_def[slot]; lazy_def[slot].clear(); }
+    };
+    long f = pm == PM_DYNAMIC ? 8L * tp->nparams : 0;  // p(t) and p - q of every parameter
     auto has = [&](const SVal& s, int j) { return j >= 0 && j <= s.deg; };
     auto pack = [&](const SVal& s) {  // full ser<K> expression
         std::string e = "ser<" + std::to_string(K) + ">{{";
@@ -358,6 +382,8 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
         if (op == OP_STOP) break;
+        for (int q = 0; q < arity(op); q++)
+            if (!(op == OP_POW_INT && q == 1)) need(I[q]);
         std::string base = "S" + std::to_string(i);
         const SVal& a = v[I[0]];
         SVal r;
@@ -474,6 +500,7 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
     std::vector<int> uset(n, 0);
     for (int i = 0; i < tp->nu; i++) {
         int row = tp->u_assign[2 * i] - 1;
+        need(tp->u_assign[2 * i + 1]);
         const SVal& s = v[tp->u_assign[2 * i + 1]];
         os << "  oKm1[" << row << "] = " << (has(s, K - 1) ? s.c[K - 1] : "mkc(0.0, 0.0)") << ";\n";
         os << "  oK[" << row << "] = " << (has(s, K) ? s.c[K] : "mkc(0.0, 0.0)") << ";\n";

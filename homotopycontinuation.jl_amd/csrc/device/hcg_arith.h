@@ -394,12 +394,33 @@ HD double h_of(double a) { return 2 * a * (sqrt(4 * (a * a) + 1) - 2 * a); }
 #define DEPS 2.220446049250313e-16
 
 // Interpreter for double-double tape evaluation (operand >= 0: register, < 0: constant -k-1)
-HD cdd dd_fetch(const double* K, const cdd* R, int e) { return e >= 0 ? R[e] : cdd_from(mkc(K[2 * (-e - 1)], K[2 * (-e - 1) + 1])); }
-__device__ __noinline__ void dd_interpret(const int* code, int ninstr, const double* K, cdd* R) {
+// Parameters of a parameter homotopy at t: p(t) = t p + (1 - t) q, and the direction
+// p - q of its Taylor series (tp!, src/homotopies/parameter_homotopy.jl:66-87).
+HD cplx par_lerp(cplx t, const double* p, const double* q, int k) {
+    cplx pk = mkc(p[2 * k], p[2 * k + 1]), qk = mkc(q[2 * k], q[2 * k + 1]);
+    if (t.im == 0.0) return c_add(c_scale(t.re, pk), c_scale(1.0 - t.re, qk));
+    return c_add(c_mul(t, pk), c_mul(mkc(1.0 - t.re, -t.im), qk));
+}
+HD cplx par_diff(const double* p, const double* q, int k) {
+    return c_sub(mkc(p[2 * k], p[2 * k + 1]), mkc(q[2 * k], q[2 * k + 1]));
+}
+// Operands of the double-double interpreter: >= 0 register, < 0 constant -e-1,
+// >= DD_PBASE homotopy parameter e - DD_PBASE (formed at t when read).
+#define DD_PBASE (1 << 24)
+struct DDPar {
+    const double* p;
+    const double* q;
+    cplx t;
+};
+HD cdd dd_fetch(const double* K, const cdd* R, const DDPar& P, int e) {
+    if (e >= DD_PBASE) return cdd_from(par_lerp(P.t, P.p, P.q, e - DD_PBASE));
+    return e >= 0 ? R[e] : cdd_from(mkc(K[2 * (-e - 1)], K[2 * (-e - 1) + 1]));
+}
+__device__ __noinline__ void dd_interpret(const int* code, int ninstr, const double* K, cdd* R, DDPar P) {
     for (int i = 0; i < ninstr; i++) {
         const int* I = code + 6 * i;
         const int op = I[0];
-        cdd a = dd_fetch(K, R, I[1]), r;
+        cdd a = dd_fetch(K, R, P, I[1]), r;
         switch (op) {
         case 1: r = cdd_cb(a); break;                       // OP_CB
         case 7: r = cdd_inv(a); break;                      // OP_INV
@@ -408,20 +429,20 @@ __device__ __noinline__ void dd_interpret(const int* code, int ninstr, const dou
         case 10: r = cdd_neg(a); break;                     // OP_NEG
         case 13: r = cdd_sqr(a); break;                     // OP_SQR
         case 17: r = a; break;                              // OP_IDENTITY
-        case 18: r = cdd_add(a, dd_fetch(K, R, I[2])); break;
-        case 19: r = cdd_div(a, dd_fetch(K, R, I[2])); break;
-        case 20: r = cdd_mul(a, dd_fetch(K, R, I[2])); break;
-        case 21: r = cdd_sub(a, dd_fetch(K, R, I[2])); break;
+        case 18: r = cdd_add(a, dd_fetch(K, R, P, I[2])); break;
+        case 19: r = cdd_div(a, dd_fetch(K, R, P, I[2])); break;
+        case 20: r = cdd_mul(a, dd_fetch(K, R, P, I[2])); break;
+        case 21: r = cdd_sub(a, dd_fetch(K, R, P, I[2])); break;
         case 22: r = cdd_pow_int(a, I[2]); break;           // OP_POW_INT (exponent inline)
-        case 24: r = cdd_add3(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3])); break;
-        case 25: r = cdd_mul3(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3])); break;
-        case 26: r = cdd_muladd(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3])); break;
-        case 27: r = cdd_mulsub(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3])); break;
-        case 28: r = cdd_submul(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3])); break;
-        case 29: r = cdd_add4(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3]), dd_fetch(K, R, I[4])); break;
-        case 30: r = cdd_mul4(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3]), dd_fetch(K, R, I[4])); break;
-        case 31: r = cdd_mulmuladd(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3]), dd_fetch(K, R, I[4])); break;
-        case 32: r = cdd_mulmulsub(a, dd_fetch(K, R, I[2]), dd_fetch(K, R, I[3]), dd_fetch(K, R, I[4])); break;
+        case 24: r = cdd_add3(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3])); break;
+        case 25: r = cdd_mul3(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3])); break;
+        case 26: r = cdd_muladd(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3])); break;
+        case 27: r = cdd_mulsub(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3])); break;
+        case 28: r = cdd_submul(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3])); break;
+        case 29: r = cdd_add4(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3]), dd_fetch(K, R, P, I[4])); break;
+        case 30: r = cdd_mul4(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3]), dd_fetch(K, R, P, I[4])); break;
+        case 31: r = cdd_mulmuladd(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3]), dd_fetch(K, R, P, I[4])); break;
+        case 32: r = cdd_mulmulsub(a, dd_fetch(K, R, P, I[2]), dd_fetch(K, R, P, I[3]), dd_fetch(K, R, P, I[4])); break;
         default: r = mkcdd(mkdd(HCG_NAN, 0.0), mkdd(HCG_NAN, 0.0));
         }
         R[I[5]] = r;

@@ -28,10 +28,8 @@ struct Lane {
     const hcg_options* o;
     // ParameterHomotopy data
 #if HCG_KIND == 1
-    const double* p;
-    const double* q;
-    cplx pt[HCG_NPAR], dp[HCG_NPAR];
-    cplx tcache; int tcache_valid;
+    const double* p;  // start parameters (device memory, shared by all paths)
+    const double* q;  // this path's target parameters
 #endif
     // TrackerState
     cplx x[NN], xh[NN], xb[NN];
@@ -89,23 +87,6 @@ struct Lane {
 };
 
 // ------------------------------- homotopy -------------------------------
-#if HCG_KIND == 1
-// tp!, src/homotopies/parameter_homotopy.jl:66-87
-__device__ void H_tp(Lane& L, cplx t) {
-    if (L.tcache_valid && L.tcache.re == t.re && L.tcache.im == t.im) return;
-    L.flops += (t.im == 0.0 ? 8.0 : 16.0) * HCG_NPAR;
-    for (int i = 0; i < HCG_NPAR; i++) {
-        cplx p = mkc(L.p[2 * i], L.p[2 * i + 1]), q = mkc(L.q[2 * i], L.q[2 * i + 1]);
-        cplx pti;
-        if (t.im == 0.0) pti = c_add(c_scale(t.re, p), c_scale(1.0 - t.re, q));
-        else pti = c_add(c_mul(t, p), c_mul(mkc(1.0 - t.re, -t.im), q));
-        L.pt[i] = pti;
-        L.dp[i] = c_sub(p, q);
-    }
-    L.tcache = t;
-    L.tcache_valid = 1;
-}
-#endif
 
 __device__ __noinline__ void H_eval(Lane& L, const cplx* x, cplx t, cplx* u) {
     L.flops += HCG_FL_EVAL;
@@ -116,8 +97,7 @@ __device__ __noinline__ void H_eval(Lane& L, const cplx* x, cplx t, cplx* u) {
     cplx ts = c_mul(mkc(HCG_GAMMA_RE, HCG_GAMMA_IM), t), tt = mkc(1.0 - t.re, -t.im);
     for (int i = 0; i < NN; i++) u[i] = c_muladd(ts, u[i], c_mul(tt, u2[i]));
 #else
-    H_tp(L, t);
-    sysF_eval(x, L.pt, u);
+    sysF_eval(x, L.p, L.q, t, u);
 #endif
 }
 
@@ -132,8 +112,7 @@ __device__ __noinline__ void H_eval_dd(Lane& L, const cplx* x, cplx t, cplx* u) 
     cplx ts = c_mul(mkc(HCG_GAMMA_RE, HCG_GAMMA_IM), t), tt = mkc(1.0 - t.re, -t.im);
     for (int i = 0; i < NN; i++) u[i] = cdd_round(cdd_add(cdd_mul_c(ts, u1[i]), cdd_mul_c(tt, u2[i])));
 #else
-    H_tp(L, t);
-    sysF_eval_dd(xd, L.pt, u1);
+    sysF_eval_dd(xd, L.p, L.q, t, u1);
     for (int i = 0; i < NN; i++) u[i] = cdd_round(u1[i]);
 #endif
 }
@@ -150,8 +129,7 @@ __device__ __noinline__ void H_eval_jac(Lane& L, const cplx* x, cplx t, cplx* u,
     for (int i = 0; i < NN; i++) u[i] = c_muladd(ts, u[i], c_mul(tt, u2[i]));
     for (int k = 0; k < NN * NN; k++) U[k] = c_muladd(ts, U[k], c_mul(tt, U2[k]));
 #else
-    H_tp(L, t);
-    sysF_eval_jac(x, L.pt, u, U);
+    sysF_eval_jac(x, L.p, L.q, t, u, U);
 #endif
 }
 
@@ -177,11 +155,10 @@ __device__ __noinline__ void H_taylor(Lane& L, int K, cplx t, cplx* u) {
         u[i] = c_add(st, tg);
     }
 #else
-    H_tp(L, t);
     cplx oK1[NN];
-    if (K == 1) sysF_taylor1(L.tx0, L.pt, L.dp, oK1, u);
-    else if (K == 2) sysF_taylor2(L.tx0, L.tx1, L.pt, L.dp, oK1, u);
-    else sysF_taylor3(L.tx0, L.tx1, L.tx2, L.pt, L.dp, oK1, u);
+    if (K == 1) sysF_taylor1(L.tx0, L.p, L.q, t, oK1, u);
+    else if (K == 2) sysF_taylor2(L.tx0, L.tx1, L.p, L.q, t, oK1, u);
+    else sysF_taylor3(L.tx0, L.tx1, L.tx2, L.p, L.q, t, oK1, u);
 #endif
 }
 
@@ -1075,9 +1052,6 @@ __device__ __noinline__ void eg_init(Lane& L, double t1) {
     L.inf_logit = 0.0f;
     L.ds_plain = HCG_NAN;
     for (int i = 0; i < 8; i++) L.feat[i] = 0.0f;
-#if HCG_KIND == 1
-    L.tcache_valid = 0;
-#endif
     for (int i = 0; i < NN; i++) { L.tx0[i] = L.tx1[i] = L.tx2[i] = L.tx3[i] = mkc(0.0, 0.0); }
     tracker_init(L, mkc(t1, 0.0), mkc(0.0, 0.0));
     L.eg_code = tc_to_eg(L.code);

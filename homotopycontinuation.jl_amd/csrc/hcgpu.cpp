@@ -246,18 +246,6 @@ static int rtc_compile(const std::string& src, std::vector<char>* code) {
     return HCG_OK;
 }
 
-// Compile-only entry (no device needed): used by the build check and CPU tests.
-extern "C" int hcg_compile_only(const hcg_homotopy_desc* h, size_t* code_size) {
-    std::string src, err;
-    hcg::Flops fl;
-    if (!hcg::generate_source(h, &src, &fl, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
-    std::vector<char> code;
-    int rc = rtc_compile(src, &code);
-    if (rc) return rc;
-    if (code_size) *code_size = code.size();
-    return HCG_OK;
-}
-
 // Generated source without compiling (no device needed).
 extern "C" int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6) {
     std::string src, err;
@@ -269,40 +257,32 @@ extern "C" int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz,
     return HCG_OK;
 }
 
-extern "C" int hcg_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* h, hcg_program** out) {
-    if (!ctx || !h || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
-    HIPCHK(hipSetDevice(ctx->device));
-    hcg_program* p = new hcg_program();
-    p->ctx = ctx;
-    p->n = h->n;
-    p->kind = h->kind;
-    p->npar = h->kind == HCG_PARAMETER ? h->target.eval->nparams : 0;
-    std::string err;
-    if (!hcg::generate_source(h, &p->source, &p->flops, &err)) { delete p; return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str()); }
-    std::vector<char> code;
-    int rc = rtc_compile(p->source, &code);
-    if (rc) { delete p; return rc; }
-    hipError_t e = hipModuleLoadData(&p->module, code.data());
-    if (e != hipSuccess) { delete p; return fail(HCG_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e)); }
-    e = hipModuleGetFunction(&p->kernel, p->module, "hcg_track_kernel");
-    if (e != hipSuccess) { hipModuleUnload(p->module); delete p; return fail(HCG_ERR_HIP, "hipModuleGetFunction: %s", hipGetErrorString(e)); }
-    *out = p;
-    return HCG_OK;
-}
+// Default register budget of the tracker kernels: waves per SIMD they are built for.
+static const int kDefaultWavesPerEU = 4;
 
-static int multi_descs_ok(const hcg_homotopy_desc* const* hs, int32_t nsys) {
+static int build_source(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi, const hcg_build_options* bo,
+                        std::string* src, std::vector<hcg::Flops>* fl) {
     if (!hs || nsys < 1 || nsys > 8) return fail(HCG_ERR_INVALID_ARGUMENT, "between 1 and 8 homotopies per program");
     for (int s = 0; s < nsys; s++)
         if (!hs[s]) return fail(HCG_ERR_INVALID_ARGUMENT, "homotopy %d is NULL", s);
+    if (nsys > 1 && !multi) return fail(HCG_ERR_INVALID_ARGUMENT, "several homotopies need a multi-system program");
+    int w = bo && bo->waves_per_eu ? bo->waves_per_eu : kDefaultWavesPerEU;
+    if (w != 1 && w != 2 && w != 4 && w != 8) return fail(HCG_ERR_INVALID_ARGUMENT, "waves_per_eu must be 1, 2, 4 or 8");
+    std::string body, err;
+    fl->assign(nsys, hcg::Flops());
+    bool ok = multi ? hcg::generate_multi_source(hs, nsys, &body, fl->data(), &err)
+                    : hcg::generate_source(hs[0], &body, &(*fl)[0], &err);
+    if (!ok) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    *src = "#define HCG_WAVES_PER_EU " + std::to_string(w) + "\n" + body;
     return HCG_OK;
 }
 
-extern "C" int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, size_t* code_size) {
-    int rc = multi_descs_ok(hs, nsys);
+extern "C" int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+                              const hcg_build_options* bo, size_t* code_size) {
+    std::string src;
+    std::vector<hcg::Flops> fl;
+    int rc = build_source(hs, nsys, multi, bo, &src, &fl);
     if (rc) return rc;
-    std::string src, err;
-    std::vector<hcg::Flops> fl(nsys);
-    if (!hcg::generate_multi_source(hs, nsys, &src, fl.data(), &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
     std::vector<char> code;
     rc = rtc_compile(src, &code);
     if (rc) return rc;
@@ -310,36 +290,54 @@ extern "C" int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_
     return HCG_OK;
 }
 
-extern "C" int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys,
-                                        hcg_program** out) {
+extern "C" int hcg_program_create_ex(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+                                     const hcg_build_options* bo, hcg_program** out) {
     if (!ctx || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
-    int rc = multi_descs_ok(hs, nsys);
-    if (rc) return rc;
     HIPCHK(hipSetDevice(ctx->device));
     hcg_program* p = new hcg_program();
     p->ctx = ctx;
-    p->nsys = nsys;
-    for (int s = 0; s < nsys; s++) {
-        p->ns.push_back(hs[s]->n);
-        p->kinds.push_back(hs[s]->kind);
-        p->npars.push_back(hs[s]->kind == HCG_PARAMETER && hs[s]->target.eval ? hs[s]->target.eval->nparams : 0);
-    }
-    p->n = *std::max_element(p->ns.begin(), p->ns.end());
-    std::string err;
-    std::vector<hcg::Flops> fl(nsys);
-    if (!hcg::generate_multi_source(hs, nsys, &p->source, fl.data(), &err)) {
-        delete p;
-        return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    std::vector<hcg::Flops> fl;
+    int rc = build_source(hs, nsys, multi, bo, &p->source, &fl);
+    if (rc) { delete p; return rc; }
+    if (multi) {
+        p->nsys = nsys;
+        for (int s = 0; s < nsys; s++) {
+            p->ns.push_back(hs[s]->n);
+            p->kinds.push_back(hs[s]->kind);
+            p->npars.push_back(hs[s]->kind == HCG_PARAMETER ? hs[s]->target.eval->nparams : 0);
+        }
+        p->n = *std::max_element(p->ns.begin(), p->ns.end());
+    } else {
+        p->n = hs[0]->n;
+        p->kind = hs[0]->kind;
+        p->npar = hs[0]->kind == HCG_PARAMETER ? hs[0]->target.eval->nparams : 0;
+        p->flops = fl[0];
     }
     std::vector<char> code;
     rc = rtc_compile(p->source, &code);
     if (rc) { delete p; return rc; }
     hipError_t e = hipModuleLoadData(&p->module, code.data());
     if (e != hipSuccess) { delete p; return fail(HCG_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e)); }
-    e = hipModuleGetFunction(&p->kernel, p->module, "hcg_track_multi_kernel");
+    e = hipModuleGetFunction(&p->kernel, p->module, multi ? "hcg_track_multi_kernel" : "hcg_track_kernel");
     if (e != hipSuccess) { hipModuleUnload(p->module); delete p; return fail(HCG_ERR_HIP, "hipModuleGetFunction: %s", hipGetErrorString(e)); }
     *out = p;
     return HCG_OK;
+}
+
+// Compile-only entries (no device needed): the build check and CPU tests use them.
+extern "C" int hcg_compile_only(const hcg_homotopy_desc* h, size_t* code_size) {
+    return hcg_compile_ex(&h, 1, 0, nullptr, code_size);
+}
+extern "C" int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, size_t* code_size) {
+    return hcg_compile_ex(hs, nsys, 1, nullptr, code_size);
+}
+extern "C" int hcg_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* h, hcg_program** out) {
+    if (!h) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    return hcg_program_create_ex(ctx, &h, 1, 0, nullptr, out);
+}
+extern "C" int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys,
+                                        hcg_program** out) {
+    return hcg_program_create_ex(ctx, hs, nsys, 1, nullptr, out);
 }
 
 extern "C" void hcg_program_destroy(hcg_program* p) {

@@ -123,6 +123,10 @@ def default_options(**overrides) -> hcg_options:
     return o
 
 
+class hcg_build_options(ctypes.Structure):
+    _fields_ = [("waves_per_eu", _I), ("reserved", _I * 7)]
+
+
 class hcg_batch(ctypes.Structure):
     _fields_ = [
         ("npaths", _I), ("ngroups", _I),
@@ -205,5 +209,5 @@ HCGPU_SYMBOLS = [
     "hcg_default_options", "hcg_track_batch", "hcg_last_kernel_stats", "hcg_mlp_load",
     "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error", "hcg_compile_only", "hcg_generate",
     "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
-    "hcg_track_multi",
+    "hcg_track_multi", "hcg_program_create_ex", "hcg_compile_ex",
 ]

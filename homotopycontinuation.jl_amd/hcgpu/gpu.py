@@ -24,19 +24,19 @@ class GpuContext:
     def handle(self):
         return self._h
 
-    def program(self, homotopy) -> "Program":
-        key = id(homotopy)
+    def program(self, homotopy, waves_per_eu: int = 0) -> "Program":
+        key = (id(homotopy), waves_per_eu)
         p = self._programs.get(key)
         if p is None:
-            p = Program(self, homotopy)
+            p = Program(self, homotopy, waves_per_eu)
             self._programs[key] = p
         return p
 
-    def program_multi(self, homotopies) -> "MultiProgram":
-        key = tuple(id(h) for h in homotopies)
+    def program_multi(self, homotopies, waves_per_eu: int = 0) -> "MultiProgram":
+        key = (tuple(id(h) for h in homotopies), waves_per_eu)
         p = self._programs.get(key)
         if p is None:
-            p = MultiProgram(self, list(homotopies))
+            p = MultiProgram(self, list(homotopies), waves_per_eu)
             self._programs[key] = p
         return p
 
@@ -77,15 +77,29 @@ def default_context() -> GpuContext:
         return _default_ctx
 
 
-class Program:
-    """A homotopy lowered to HIP and compiled with hipRTC (hcg_program_create)."""
+def _create(ctx, descs, multi: bool, waves_per_eu: int):
+    h = ctypes.c_void_p()
+    arr = (ctypes.POINTER(abi.hcg_homotopy_desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
+    bo = abi.hcg_build_options()
+    bo.waves_per_eu = waves_per_eu
+    L = lib()
+    L.hcg_program_create_ex.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(abi.hcg_homotopy_desc)),
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(abi.hcg_build_options),
+                                        ctypes.POINTER(ctypes.c_void_p)]
+    check(L.hcg_program_create_ex(ctx.handle, arr, len(descs), int(multi), ctypes.byref(bo), ctypes.byref(h)))
+    return h
 
-    def __init__(self, ctx: GpuContext, homotopy):
+
+class Program:
+    """A homotopy lowered to HIP and compiled with hipRTC (hcg_program_create_ex);
+    waves_per_eu = 0 takes the library's default register budget."""
+
+    def __init__(self, ctx: GpuContext, homotopy, waves_per_eu: int = 0):
         self.ctx = ctx
         self.homotopy = homotopy
         self.n = homotopy.n
-        self._h = ctypes.c_void_p()
-        check(lib().hcg_program_create(ctx.handle, ctypes.byref(homotopy.desc()), ctypes.byref(self._h)))
+        self.waves_per_eu = waves_per_eu
+        self._h = _create(ctx, [homotopy.desc()], False, waves_per_eu)
 
     def source(self) -> str:
         return lib().hcg_program_source(self._h).decode()
@@ -179,19 +193,14 @@ class MultiProgram:
     """Several homotopies compiled into one program (hcg_program_create_multi): one launch
     tracks the paths of all of them from a single work queue."""
 
-    def __init__(self, ctx: GpuContext, homotopies):
+    def __init__(self, ctx: GpuContext, homotopies, waves_per_eu: int = 0):
         if not 1 <= len(homotopies) <= 8:
             raise ValueError("between 1 and 8 homotopies per program")
         self.ctx = ctx
         self.homotopies = homotopies
         self.ns = [h.n for h in homotopies]
-        self._h = ctypes.c_void_p()
-        descs = [h.desc() for h in homotopies]
-        arr = (ctypes.POINTER(abi.hcg_homotopy_desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
-        L = lib()
-        L.hcg_program_create_multi.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(abi.hcg_homotopy_desc)),
-                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
-        check(L.hcg_program_create_multi(ctx.handle, arr, len(descs), ctypes.byref(self._h)))
+        self.waves_per_eu = waves_per_eu
+        self._h = _create(ctx, [h.desc() for h in homotopies], True, waves_per_eu)
 
     def close(self):
         if self._h:

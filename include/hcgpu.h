@@ -206,6 +206,20 @@ int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batc
  * no single reference call it replaces. */
 int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, hcg_program** out);
 int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, size_t* code_size);
+
+/* Build choices of a program (no counterpart in the reference; picked by the autotuner).
+ * waves_per_eu: waves per SIMD the tracker kernel is compiled for, i.e. a register budget
+ * of 512 / waves_per_eu per lane (1, 2, 4 or 8; 0 = the library default). */
+typedef struct hcg_build_options {
+    int32_t waves_per_eu;
+    int32_t reserved[7];
+} hcg_build_options;
+/* General form of hcg_program_create / hcg_program_create_multi (multi = 1: one kernel over
+ * all nsys homotopies; multi = 0 requires nsys = 1); bo may be NULL. */
+int hcg_program_create_ex(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+                          const hcg_build_options* bo, hcg_program** out);
+int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi, const hcg_build_options* bo,
+                   size_t* code_size);
 /* batches[s] describes the paths of system s.  Outputs are concatenated in system order:
  * results: sum_s npaths_s records; solutions and last_points: sum_s npaths_s * n_s complex;
  * valuations: sum_s npaths_s * n_s doubles. */

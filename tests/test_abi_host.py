@@ -55,6 +55,7 @@ STRUCTS = {
     "hcg_batch": abi.hcg_batch,
     "hcg_path_result": abi.hcg_path_result,
     "hcg_step_log": abi.hcg_step_log,
+    "hcg_build_options": abi.hcg_build_options,
 }
 
 
