@@ -111,6 +111,37 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     const char* pre = dd ? "cdd_" : "c_";
     long f = pm == PM_DYNAMIC ? 6L * tp->nparams : 0;  // p(t) = t p + (1 - t) q per parameter
     long dm = dd ? 58 : 6, da = dd ? 22 : 2;
+    // every output is stored right after the last instruction writing its slot (short live
+    // ranges); outputs read straight from inputs or constants are stored at the end
+    std::vector<int> last_write(tp->tape_space + 1, -1);
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        if (I[4] == OP_STOP) break;
+        last_write[I[5]] = i;
+    }
+    std::vector<std::vector<std::string>> stores_after(tp->ninstr + 1);
+    std::vector<int> uset(n, 0), Uset(jac ? n * n : 0, 0);
+    for (int i = 0; i < tp->nu; i++) {
+        int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
+        int at = last_write[slot] >= 0 ? last_write[slot] : tp->ninstr;
+        stores_after[at].push_back("u[" + std::to_string(row) + "]#" + std::to_string(slot));
+        uset[row] = 1;
+    }
+    if (jac)
+        for (int i = 0; i < tp->nU; i++) {
+            int l = tp->U_assign[2 * i] - 1, slot = tp->U_assign[2 * i + 1];
+            int at = last_write[slot] >= 0 ? last_write[slot] : tp->ninstr;
+            stores_after[at].push_back("U[" + std::to_string(l) + "]#" + std::to_string(slot));
+            Uset[l] = 1;
+        }
+    auto flush_stores = [&](int at) {
+        for (auto& st : stores_after[at]) {
+            size_t h = st.find('#');
+            int slot = std::stoi(st.substr(h + 1));
+            need(slot);
+            os << "  " << st.substr(0, h) << " = " << nm[slot] << ";\n";
+        }
+    };
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
@@ -146,27 +177,14 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
         }
         os << "  const " << T << " " << out << " = " << e << ";\n";
         nm[I[5]] = out;
+        flush_stores(i);
     }
-    std::vector<int> uset(n, 0);
-    for (int i = 0; i < tp->nu; i++) {
-        int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
-        need(slot);
-        os << "  u[" << row << "] = " << nm[slot] << ";\n";
-        uset[row] = 1;
-    }
+    flush_stores(tp->ninstr);
     for (int i = 0; i < n; i++)
         if (!uset[i]) os << "  u[" << i << "] = " << (dd ? "cdd_from(mkc(0.0, 0.0))" : "mkc(0.0, 0.0)") << ";\n";
-    if (jac) {
-        std::vector<int> Uset(n * n, 0);
-        for (int i = 0; i < tp->nU; i++) {
-            int l = tp->U_assign[2 * i] - 1, slot = tp->U_assign[2 * i + 1];
-            need(slot);
-            os << "  U[" << l << "] = " << nm[slot] << ";\n";
-            Uset[l] = 1;
-        }
+    if (jac)
         for (int i = 0; i < n * n; i++)
             if (!Uset[i]) os << "  U[" << i << "] = mkc(0.0, 0.0);\n";
-    }
     os << "}\n";
     *flops = f;
 }
@@ -378,6 +396,28 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         }
         return e + "}}";
     };
+    // outputs are stored right after the last instruction writing their slot
+    std::vector<int> last_write(tp->tape_space + 1, -1);
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        if (I[4] == OP_STOP) break;
+        last_write[I[5]] = i;
+    }
+    std::vector<std::vector<std::pair<int, int>>> stores_after(tp->ninstr + 1);  // (row, slot)
+    std::vector<int> uset(n, 0);
+    for (int i = 0; i < tp->nu; i++) {
+        int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
+        stores_after[last_write[slot] >= 0 ? last_write[slot] : tp->ninstr].push_back({row, slot});
+        uset[row] = 1;
+    }
+    auto flush_stores = [&](int at) {
+        for (auto& rs : stores_after[at]) {
+            need(rs.second);
+            const SVal& sv = v[rs.second];
+            os << "  oKm1[" << rs.first << "] = " << (has(sv, K - 1) ? sv.c[K - 1] : "mkc(0.0, 0.0)") << ";\n";
+            os << "  oK[" << rs.first << "] = " << (has(sv, K) ? sv.c[K] : "mkc(0.0, 0.0)") << ";\n";
+        }
+    };
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
@@ -496,16 +536,9 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         }
         }
         v[I[5]] = r;
+        flush_stores(i);
     }
-    std::vector<int> uset(n, 0);
-    for (int i = 0; i < tp->nu; i++) {
-        int row = tp->u_assign[2 * i] - 1;
-        need(tp->u_assign[2 * i + 1]);
-        const SVal& s = v[tp->u_assign[2 * i + 1]];
-        os << "  oKm1[" << row << "] = " << (has(s, K - 1) ? s.c[K - 1] : "mkc(0.0, 0.0)") << ";\n";
-        os << "  oK[" << row << "] = " << (has(s, K) ? s.c[K] : "mkc(0.0, 0.0)") << ";\n";
-        uset[row] = 1;
-    }
+    flush_stores(tp->ninstr);
     for (int i = 0; i < n; i++)
         if (!uset[i]) os << "  oKm1[" << i << "] = mkc(0.0, 0.0);\n  oK[" << i << "] = mkc(0.0, 0.0);\n";
     os << "}\n";

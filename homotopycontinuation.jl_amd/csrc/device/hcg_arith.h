@@ -396,10 +396,14 @@ HD double h_of(double a) { return 2 * a * (sqrt(4 * (a * a) + 1) - 2 * a); }
 // Interpreter for double-double tape evaluation (operand >= 0: register, < 0: constant -k-1)
 // Parameters of a parameter homotopy at t: p(t) = t p + (1 - t) q, and the direction
 // p - q of its Taylor series (tp!, src/homotopies/parameter_homotopy.jl:66-87).
+// Both forms are computed and one is selected (no branch): inlined for every parameter of
+// a straight-line body, a branch would split it into hundreds of blocks.
 HD cplx par_lerp(cplx t, const double* p, const double* q, int k) {
     cplx pk = mkc(p[2 * k], p[2 * k + 1]), qk = mkc(q[2 * k], q[2 * k + 1]);
-    if (t.im == 0.0) return c_add(c_scale(t.re, pk), c_scale(1.0 - t.re, qk));
-    return c_add(c_mul(t, pk), c_mul(mkc(1.0 - t.re, -t.im), qk));
+    cplx rr = c_add(c_scale(t.re, pk), c_scale(1.0 - t.re, qk));
+    cplx rc = c_add(c_mul(t, pk), c_mul(mkc(1.0 - t.re, -t.im), qk));
+    const bool real_t = t.im == 0.0;
+    return mkc(real_t ? rr.re : rc.re, real_t ? rr.im : rc.im);
 }
 HD cplx par_diff(const double* p, const double* q, int k) {
     return c_sub(mkc(p[2 * k], p[2 * k + 1]), mkc(q[2 * k], q[2 * k + 1]));

@@ -39,6 +39,8 @@ ARITY = {
     "STOP": 0,
 }
 
+_ARITY_OF = {OP[k]: v for k, v in ARITY.items()}
+
 
 class hcg_tape(ctypes.Structure):
     _fields_ = [
@@ -187,6 +189,45 @@ class _Builder:
             acc = const_ref if acc is None else self.emit("ADD", acc, const_ref)
         return acc
 
+    def polys(self, fs: List[Poly]):
+        """MULADD chains of several polynomials built in one sweep over their monomials: for
+        every monomial (graded order) each polynomial containing it takes its term next.  Each
+        polynomial's chain is exactly the one poly() builds (same terms in the same order), but
+        a shared monomial is consumed right after it is formed instead of staying live until
+        the last polynomial, which keeps the generated straight-line code's register pressure
+        low.  Returns one reference (or None) per polynomial."""
+        nv = self.nvars
+        groups: List[Dict[Tuple[int, ...], Dict[Tuple[int, ...], complex]]] = []
+        for f in fs:
+            g: Dict[Tuple[int, ...], Dict[Tuple[int, ...], complex]] = {}
+            for e, c in f.terms.items():
+                g.setdefault(e[:nv], {})[e[nv:]] = c
+            groups.append(g)
+        order = lambda a: (sum(a), tuple(-k for k in a))
+        keys = sorted({k for g in groups for k in g}, key=order)
+        acc: List[object] = [None] * len(fs)
+        const_ref: List[object] = [None] * len(fs)
+        for xe in keys:
+            for r, g in enumerate(groups):
+                if xe not in g:
+                    continue
+                cref = self._coeff(g[xe])
+                if cref is None:
+                    continue
+                if sum(xe) == 0:
+                    const_ref[r] = cref
+                    continue
+                m = self.monomial(xe + (0,) * self.nparams)
+                one = cref[0] == "c" and self.consts[cref[1]] == 1
+                if acc[r] is None:
+                    acc[r] = m if one else self.emit("MUL", cref, m)
+                else:
+                    acc[r] = self.emit("ADD", m, acc[r]) if one else self.emit("MULADD", cref, m, acc[r])
+        for r in range(len(fs)):
+            if const_ref[r] is not None:
+                acc[r] = const_ref[r] if acc[r] is None else self.emit("ADD", acc[r], const_ref[r])
+        return acc
+
     def _coeff(self, cp: Dict[Tuple[int, ...], complex]):
         """Reference to a coefficient that is a polynomial in the parameters."""
         cp = {e: c for e, c in cp.items() if c != 0}
@@ -244,21 +285,89 @@ class _Builder:
         return Tape(rows, self.consts, self.nparams, self.nvars, u_assign, U_assign, output_dim)
 
 
+def max_live(t: Tape) -> int:
+    """Peak number of simultaneously live complex values of the straight-line code generated
+    from `t` (inputs live from first to last use, every output stored right after the last
+    instruction writing it; constants are literals)."""
+    rows = [r for r in t.instr if r[4] != OP["STOP"]]
+    first_input = t.nconst + 1
+    last_use: Dict[int, int] = {}
+    first_use: Dict[int, int] = {}
+    for i, r in enumerate(rows):
+        ar = _ARITY_OF[int(r[4])]
+        for q in range(ar):
+            if r[4] == OP["POW_INT"] and q == 1:
+                continue
+            sl = r[q]
+            if sl >= first_input:
+                last_use[sl] = i
+                first_use.setdefault(sl, i)
+    for _, sl in list(t.u_assign) + list(t.U_assign):
+        last_use.setdefault(sl, len(rows))
+    start = {}
+    for i, r in enumerate(rows):
+        start.setdefault(r[5], i)
+    delta = [0] * (len(rows) + 2)
+    for sl, lu in last_use.items():
+        b0 = start.get(sl, first_use.get(sl, lu))
+        delta[b0] += 1
+        delta[lu + 1] -= 1
+    live = peak = 0
+    for d in delta:
+        live += d
+        peak = max(peak, live)
+    return peak
+
+
+def _best(variants):
+    """The tape with the smallest register pressure among equivalent emission orders (peak
+    live values, with recomputation charged at one live value per 50 extra instructions)."""
+    tapes_ = [make() for make in variants]
+    nmin = min(len(t.instr) for t in tapes_)
+    scored = [(max_live(t) + (len(t.instr) - nmin) / 50.0, k, t) for k, t in enumerate(tapes_)]
+    return min(scored, key=lambda x: (x[0], x[1]))[2]
+
+
 def eval_tape(system: System) -> Tape:
-    """The evaluate! tape: u = F(x; p)."""
-    b = _Builder(system.nvars, system.nparams)
-    outs = [b.poly(f) for f in system.polys]
-    return b.finish(outs, None, len(system.polys))
+    """The evaluate! tape: u = F(x; p).  The polynomials are emitted either one after the
+    other or interleaved over their shared monomials, whichever keeps fewer values live;
+    both orders compute every output with exactly the same operations."""
+    def interleaved():
+        b = _Builder(system.nvars, system.nparams)
+        return b.finish(b.polys(list(system.polys)), None, len(system.polys))
+
+    def sequential():
+        b = _Builder(system.nvars, system.nparams)
+        return b.finish([b.poly(f) for f in system.polys], None, len(system.polys))
+
+    return _best([interleaved, sequential])
 
 
 def jacobian_tape(system: System) -> Tape:
     """The evaluate_and_jacobian! tape: u = F, U = dF/dx (jacobian_instruction_sequence,
-    src/model_kit/instruction_sequence.jl:256-272)."""
-    b = _Builder(system.nvars, system.nparams)
-    outs = [b.poly(f) for f in system.polys]
+    src/model_kit/instruction_sequence.jl:256-272).  Emission order as in eval_tape: all
+    n + n^2 polynomials interleaved, each row (f_i and its gradient) interleaved, or one
+    polynomial after the other, whichever keeps fewer values live."""
+    m, n = len(system.polys), system.nvars
     J = system.jacobian()
-    jac = [[b.poly(J[i][j]) for j in range(system.nvars)] for i in range(len(system.polys))]
-    return b.finish(outs, jac, len(system.polys))
+
+    def build(groups, remat=False):
+        b = _Builder(system.nvars, system.nparams)
+        refs = {}
+        for g in groups:
+            if remat:  # recompute (bit-identical) monomials per group instead of keeping them live
+                b.mono = {e: r for e, r in b.mono.items() if sum(e) == 1}
+            for key, r in zip(g, b.polys([system.polys[i] if j < 0 else J[i][j] for i, j in g])):
+                refs[key] = r
+        outs = [refs[(i, -1)] for i in range(m)]
+        jac = [[refs[(i, j)] for j in range(n)] for i in range(m)]
+        return b.finish(outs, jac, m)
+
+    everything = [[(i, -1) for i in range(m)] + [(i, j) for i in range(m) for j in range(n)]]
+    by_row = [[(i, -1)] + [(i, j) for j in range(n)] for i in range(m)]
+    one_by_one = [[(i, -1)] for i in range(m)] + [[(i, j)] for i in range(m) for j in range(n)]
+    return _best([lambda: build(everything), lambda: build(by_row), lambda: build(one_by_one),
+                  lambda: build(by_row, remat=True)])
 
 
 def tapes(system: System) -> Tuple[Tape, Tape]:
