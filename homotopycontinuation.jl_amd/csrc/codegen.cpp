@@ -55,6 +55,9 @@ bool validate_tape(const hcg_tape* tp, int n, std::string* err) {
     if (tp->nvars != n) { *err = "tape variable count differs from the homotopy size"; return false; }
     if (tp->output_dim != n) { *err = "only square systems are supported"; return false; }
     if (tp->nconst < 0 || tp->nparams < 0 || tp->tape_space < tp->nconst + tp->nparams + tp->nvars) { *err = "bad tape ranges"; return false; }
+    if (tp->ninstr < 0 || tp->nu < 0 || tp->nU < 0 || (tp->nu > 0 && !tp->u_assign) || (tp->nU > 0 && !tp->U_assign) ||
+        (tp->nconst > 0 && !tp->constants)) { *err = "bad tape counts"; return false; }
+    if (tp->tape_space > (1 << 22) || tp->ninstr > (1 << 22)) { *err = "tape too large"; return false; }
     if (tp->params_first != tp->nconst + 1 || tp->vars_first != tp->nconst + tp->nparams + 1) { *err = "unexpected tape block layout"; return false; }
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;

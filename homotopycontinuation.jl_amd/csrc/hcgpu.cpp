@@ -247,7 +247,7 @@ static int rtc_compile(const std::string& src, std::vector<char>* code) {
 }
 
 // Generated source without compiling (no device needed).
-extern "C" int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6) {
+static int hcg_generate_impl(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6) {
     std::string src, err;
     hcg::Flops fl;
     if (!hcg::generate_source(h, &src, &fl, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
@@ -277,7 +277,7 @@ static int build_source(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_
     return HCG_OK;
 }
 
-extern "C" int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+static int hcg_compile_ex_impl(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
                               const hcg_build_options* bo, size_t* code_size) {
     std::string src;
     std::vector<hcg::Flops> fl;
@@ -290,7 +290,7 @@ extern "C" int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, 
     return HCG_OK;
 }
 
-extern "C" int hcg_program_create_ex(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+static int hcg_program_create_ex_impl(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
                                      const hcg_build_options* bo, hcg_program** out) {
     if (!ctx || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     HIPCHK(hipSetDevice(ctx->device));
@@ -444,7 +444,7 @@ static int finish_launch(hcg_ctx* ctx) {
     return HCG_OK;
 }
 
-extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+static int hcg_track_batch_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
                                hcg_path_result* results, double* solutions, double* last_points, double* valuations,
                                hcg_step_log* log) {
     int rc = validate_batch_args(ctx, prog, b, opts, true);
@@ -505,7 +505,7 @@ extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_
     return HCG_OK;
 }
 
-extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+static int hcg_track_batch_device_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
                                       const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
                                       double* last_points_dev, double* valuations_dev) {
     int rc = validate_batch_args(ctx, prog, batch_dev, opts, false);
@@ -520,7 +520,7 @@ extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, con
     return finish_launch(ctx);
 }
 
-extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches,
+static int hcg_track_multi_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches,
                                const hcg_options* opts, hcg_path_result* results, double* solutions,
                                double* last_points, double* valuations) {
     if (!ctx || !prog || !batches || !opts || !results || !solutions || !last_points || !valuations)
@@ -680,4 +680,60 @@ extern "C" int hcg_mlp_train(hcg_ctx* ctx, float* params, int32_t n, const float
     int rc = hcg::mlp_train(&ctx->mlp_work, ctx->stream, params, n, x, y, w, epochs, batch, lr, seed, final_loss, &err);
     if (rc) return fail(rc, "%s", err.c_str());
     return HCG_OK;
+}
+
+// ---- C ABI: no C++ exception crosses the boundary ----
+extern "C" int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6) {
+    try {
+        return hcg_generate_impl(h, buf, bufsz, needed, flops6);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_generate: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_generate: unexpected error");
+    }
+}
+extern "C" int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi, const hcg_build_options* bo, size_t* code_size) {
+    try {
+        return hcg_compile_ex_impl(hs, nsys, multi, bo, code_size);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_compile_ex: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_compile_ex: unexpected error");
+    }
+}
+extern "C" int hcg_program_create_ex(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi, const hcg_build_options* bo, hcg_program** out) {
+    try {
+        return hcg_program_create_ex_impl(ctx, hs, nsys, multi, bo, out);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_program_create_ex: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_program_create_ex: unexpected error");
+    }
+}
+extern "C" int hcg_track_batch(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts, hcg_path_result* results, double* solutions, double* last_points, double* valuations, hcg_step_log* log) {
+    try {
+        return hcg_track_batch_impl(ctx, prog, b, opts, results, solutions, last_points, valuations, log);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch: unexpected error");
+    }
+}
+extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev, const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev, double* last_points_dev, double* valuations_dev) {
+    try {
+        return hcg_track_batch_device_impl(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device: unexpected error");
+    }
+}
+extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches, const hcg_options* opts, hcg_path_result* results, double* solutions, double* last_points, double* valuations) {
+    try {
+        return hcg_track_multi_impl(ctx, prog, batches, opts, results, solutions, last_points, valuations);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_multi: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_multi: unexpected error");
+    }
 }
