@@ -5,8 +5,9 @@
 // on, an MLP 8 -> 64 -> 64 (GELU) -> 2 runs per lane in FP32 from BF16 weights staged in
 // LDS.  Output 0 is the log of the step relative to the plain rule's proposal; the
 // proposal is only ever lengthened, by at most options.learned_max_mult, and stays under
-// the plain rule's growth limit.  Output 1 is the at-infinity logit (logged, not acted
-// on).  A learned step must pass the same Newton acceptance test as a plain one; after a
+// the plain rule's growth limit.  Output 1 is the at-infinity logit: when options
+// learned_inf_t > 0, a path whose logit exceeds learned_inf_logit starts its endgame once
+// t <= learned_inf_t (the endgame's own valuation checks still decide every path).  A learned step must pass the same Newton acceptance test as a plain one; after a
 // rejected step the plain rule alone proposes until a step is accepted again, so the
 // plain rule remains both the fallback and the bound.
 // (Included once per system by hcg_tracker_body.h.)
@@ -35,6 +36,12 @@ __device__ double learned_adjust(Lane& L, const NewtRes& res, double ds) {
     float out[2];
     mlp_forward(L.mlp_w, L.feat, out);
     L.inf_logit = out[1];
+    {   // at-infinity flag: start the endgame (its own checks decide) at a larger t
+        const hcg_options* o = L.o;
+        double tn = fabs(seg_t(L).re);
+        if (o->learned_inf_t > 0.0 && out[1] > o->learned_inf_logit && tn <= o->learned_inf_t && tn > L.eg_start)
+            L.eg_start = tn;
+    }
     double lm = jclamp((double)out[0], 0.0, hc_log(L.o->learned_max_mult));
     if (!(lm > 0.0)) return ds;
     double dsl = jmin(jmin(ds * hc_exp(lm), 10 * L.ds_prev), L.o->max_step_size);

@@ -76,6 +76,7 @@ struct Lane {
     // learned proposer bookkeeping
     int learned_proposed, learned_accepted, last_learned;
     double cond_H_xd, ds_plain;
+    double eg_start;  // endgame_start, or an earlier start for paths flagged as heading to infinity
     float feat[8], inf_logit;
     const unsigned short* mlp_w;  // BF16 weights in LDS, or null (plain rule only)
     double flops;  // counted FP64 flops of this path
@@ -1051,6 +1052,7 @@ __device__ __noinline__ void eg_init(Lane& L, double t1) {
     L.last_learned = 0;
     L.inf_logit = 0.0f;
     L.ds_plain = HCG_NAN;
+    L.eg_start = L.o->endgame_start;
     for (int i = 0; i < 8; i++) L.feat[i] = 0.0f;
     for (int i = 0; i < NN; i++) { L.tx0[i] = L.tx1[i] = L.tx2[i] = L.tx3[i] = mkc(0.0, 0.0); }
     tracker_init(L, mkc(t1, 0.0), mkc(0.0, 0.0));
@@ -1294,7 +1296,7 @@ __device__ __noinline__ void eg_step(Lane& L) {
     if (L.eg_code != EG_TRACKING) { tracking_stopped(L); return; }
     L.jtzf0 = L.jtzf1; L.jtzf1 = jump;
     double t = seg_t(L).re;
-    if (!(t <= o->endgame_start)) return;
+    if (!(t <= L.eg_start)) return;
     if (L.steps_eg == 0) L.ext_steps_eg_start = ext_steps(L);
     L.steps_eg += 1;
     if (!ok) return;
@@ -1319,7 +1321,7 @@ __device__ __noinline__ void write_result(Lane& L, const hcg_kargs& a, int k) {
     R.accepted_steps = L.accepted;
     R.rejected_steps = L.rejected;
     R.steps_eg = L.steps_eg;
-    R.valuation_valid = !(t > o->endgame_start);
+    R.valuation_valid = !(t > L.eg_start);
     R.newton_iters = L.newton_iters;
     R.learned_proposed = L.learned_proposed;
     R.learned_accepted = L.learned_accepted;

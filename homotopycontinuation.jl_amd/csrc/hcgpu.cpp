@@ -71,6 +71,7 @@ struct hcg_program {
     hipModule_t module = nullptr;
     hipFunction_t kernel = nullptr;
     hcg::Flops flops;
+    int waves_per_eu = 1;  // register budget the kernel was built for (waves per SIMD)
     // multi-system programs (hcg_program_create_multi): per-system size, kind, parameters
     int nsys = 0;
     std::vector<int> ns, kinds, npars;
@@ -136,6 +137,7 @@ extern "C" void hcg_default_options(hcg_options* o) {
     o->sing_cond = 1e14; o->sing_accuracy = 1e-12; o->scaling_threshold = -30.0;
     o->scale_min = 1e-4; o->scale_abs_min = 1e-6; o->scale_max = std::ldexp(1.0, 511);
     o->learned_max_mult = 4.0; o->learned_min_t = 0.0;
+    o->learned_inf_t = 0.0; o->learned_inf_logit = 4.0;
 }
 
 // ------------------------------ hipRTC ------------------------------
@@ -299,6 +301,7 @@ static int hcg_program_create_ex_impl(hcg_ctx* ctx, const hcg_homotopy_desc* con
     std::vector<hcg::Flops> fl;
     int rc = build_source(hs, nsys, multi, bo, &p->source, &fl);
     if (rc) { delete p; return rc; }
+    p->waves_per_eu = bo && bo->waves_per_eu ? bo->waves_per_eu : kDefaultWavesPerEU;
     if (multi) {
         p->nsys = nsys;
         for (int s = 0; s < nsys; s++) {
@@ -423,7 +426,7 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
     ka.log_count = d_logc;
     ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
     const int block = 64;
-    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
     long waves_needed = ((long)b->npaths + block - 1) / block;
     long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
     if (grid < 1) grid = 1;
@@ -621,7 +624,7 @@ static int hcg_track_multi_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg
         }
     m.offset[S] = (int32_t)total;
     const int block = 64;
-    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 8;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
     long grid = std::min<long>((total + block - 1) / block, (long)ctx->ncu * per_cu);
     if (grid < 1) grid = 1;
     void* args[] = {&m};

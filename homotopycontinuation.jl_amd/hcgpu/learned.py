@@ -175,3 +175,17 @@ def fit_from_log(ctx, log, path_codes, epochs: int = 8, batch: int = 4096, lr: f
     p, loss = train(ctx, p0, Xn, Y, W, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return fold_scaling(p, mu, sd), loss, dict(records=len(X), weighted=int(W[:, 0].sum()),
                                                  mean_target=float(Y[W[:, 0] > 0, 0].mean()) if W[:, 0].any() else 0.0)
+
+
+def combine_logs(parts):
+    """Concatenate (step log, path codes) pairs of several launches into one training log:
+    path indices of later parts are offset so that every path stays distinct."""
+    logs, codes = [], []
+    off = 0
+    for log, c in parts:
+        L = np.array(log, dtype=np.float32, copy=True)
+        L[:, 0] += off
+        logs.append(L)
+        codes.append(np.asarray(c))
+        off += len(c)
+    return np.concatenate(logs), np.concatenate(codes)

@@ -124,6 +124,10 @@ typedef struct hcg_options {
     int32_t log_steps;        /* 1: write per-step feature records into hcg_step_log */
     double learned_max_mult;  /* largest factor a learned proposal may lengthen the plain one by */
     double learned_min_t;     /* learned proposals only while |t| > learned_min_t */
+    double learned_inf_t;     /* > 0: a path the network flags as heading to infinity (logit above
+                                 learned_inf_logit) starts its endgame once t <= learned_inf_t instead
+                                 of endgame_start; 0 = the flag is not used */
+    double learned_inf_logit;
 } hcg_options;
 
 /* A batch of paths for one program. */

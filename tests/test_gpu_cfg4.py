@@ -81,3 +81,29 @@ def test_full_size_one_launch(gpu_ctx, probs):
             assert _residuals(H.F.system, X).max() < 1e-11
             key = np.round(X[:, :3], 6)
             assert len({tuple(r) for r in key}) == len(X), f"n={H.n}: repeated finite endpoints"
+
+
+def test_learned_mixed_same_results(gpu_ctx, probs):
+    """The mixed workload with the learned proposer (network trained on shared-factor
+    systems disjoint from these): every path ends with the plain tracker's return code, finite
+    endpoints agree within 1e-10, and the batch needs fewer steps."""
+    from hcgpu import learned
+    w = learned.saved_weights("mixed_step_mlp")
+    assert w is not None, "hcgpu/weights/mixed_step_mlp.npy is missing (train_mixed.py writes it)"
+    Hs = [H for H, _ in probs]
+    starts = [S if H.n < 12 else S[:32768] for H, S in probs]
+    multi = gpu_ctx.program_multi(Hs)
+    plain = track_multi(multi, starts, make_options())
+    learned.load(gpu_ctx, w)
+    outs = track_multi(multi, starts, make_options(learned_step=1))
+    sp = sl = 0
+    for H, g, p in zip(Hs, outs, plain):
+        assert list(g.code) == list(p.code), f"n={H.n}"
+        ok = (p.code == 1) & (~p.singular)
+        if ok.any():
+            rel = np.abs(g.solution[ok] - p.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(p.solution[ok]).max(axis=1))
+            assert rel.max() <= 1e-10, (H.n, rel.max())
+        sp += int((p.accepted_steps + p.rejected_steps).sum())
+        sl += int((g.accepted_steps + g.rejected_steps).sum())
+    print(f"steps plain {sp} learned {sl}")
+    assert sl < 0.85 * sp
