@@ -169,12 +169,21 @@ __device__ __forceinline__ void ws_updated(Lane& L) {
     for (int k = 0; k < NN * NN; k++) L.LU[k] = L.A[k];
 }
 // lu!, src/linear_algebra.jl:130-184
+// Loops over the compile-time size are unrolled for small systems, so that the loads of a
+// column are issued together (the matrices live in the lane's private memory).
+#if NN <= 12
+#define HCG_UNROLL _Pragma("unroll")
+#else
+#define HCG_UNROLL
+#endif
 __device__ __noinline__ void lu_factor(Lane& L) {
     cplx* A = L.LU;
     L.flops += FL_LU;
+    HCG_UNROLL
     for (int k = 0; k < NN; k++) {
         int kp = k;
         double amax = c_abs2(A[IDX(k, k)]);
+        HCG_UNROLL
         for (int i = k + 1; i < NN; i++) {
             double absi = c_abs2(A[IDX(i, k)]);
             if (absi > amax) { kp = i; amax = absi; }
@@ -184,10 +193,13 @@ __device__ __noinline__ void lu_factor(Lane& L) {
             if (k != kp)
                 for (int i = 0; i < NN; i++) { cplx t = A[IDX(k, i)]; A[IDX(k, i)] = A[IDX(kp, i)]; A[IDX(kp, i)] = t; }
             cplx Akk = A[IDX(k, k)];
+            HCG_UNROLL
             for (int i = k + 1; i < NN; i++) A[IDX(i, k)] = c_div(A[IDX(i, k)], Akk);
         }
+        HCG_UNROLL
         for (int j = k + 1; j < NN; j++) {
             cplx Akj = A[IDX(k, j)];
+            HCG_UNROLL
             for (int i = k + 1; i < NN; i++) A[IDX(i, j)] = c_submul(A[IDX(i, k)], Akj, A[IDX(i, j)]);
         }
     }
@@ -198,13 +210,17 @@ __device__ __noinline__ void lu_solve(Lane& L, cplx* x) {
     const cplx* A = L.LU;
     L.flops += FL_SOLVE;
     for (int i = 0; i < NN; i++) { int p = L.ipiv[i]; if (i != p) { cplx t = x[i]; x[i] = x[p]; x[p] = t; } }
+    HCG_UNROLL
     for (int j = 0; j < NN; j++) {
         cplx xj = x[j];
+        HCG_UNROLL
         for (int i = j + 1; i < NN; i++) x[i] = c_submul(A[IDX(i, j)], xj, x[i]);
     }
+    HCG_UNROLL
     for (int j = NN - 1; j >= 0; j--) {
         cplx xj = c_div(x[j], A[IDX(j, j)]);
         x[j] = xj;
+        HCG_UNROLL
         for (int i = 0; i < j; i++) x[i] = c_submul(A[IDX(i, j)], xj, x[i]);
     }
 }
@@ -1415,3 +1431,4 @@ __device__ __forceinline__ void lane_finish(Lane& L, const hcg_kargs& a, int k, 
 #undef FL_SOLVE
 #undef FL_MIXED
 #undef FL_FIXED
+#undef HCG_UNROLL
