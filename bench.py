@@ -227,7 +227,9 @@ def bench_dense8(args, out):
     from hcgpu.solve import make_options
     H, S = total_degree(dense_system(8, 3, seed=1), seed=2)
     ctx = GpuContext(0)
-    batch = DeviceBatch(ctx, ctx.program(H), S)
+    # 6,561 paths fill 103 waves, far fewer than the GPU's SIMDs: the kernel is built for one
+    # wave per SIMD (the largest register budget) instead of the default four
+    batch = DeviceBatch(ctx, ctx.program(H, waves_per_eu=1), S)
     r = timed(batch, make_options(), args.steps, args.warmup)
     rec, _ = batch.results()
     npaths = len(S)
@@ -249,6 +251,7 @@ def bench_dense8(args, out):
 
 def bench_mixed(args, out):
     import numpy as np
+    from hcgpu import learned
     from hcgpu.configs import cfg4_problem
     from hcgpu.gpu import GpuContext, track_multi
     from hcgpu.solve import make_options
@@ -257,21 +260,48 @@ def bench_mixed(args, out):
     prog = ctx.program_multi([H for H, _ in probs])
     starts = [S for _, S in probs]
     total = sum(len(S) for S in starts)
-    for _ in range(args.warmup):
-        track_multi(prog, starts, make_options())
-    kms, fl = [], []
-    outs = None
-    for _ in range(args.steps):
-        outs = track_multi(prog, starts, make_options())
-        kms.append(ctx.last_kernel_ms())
-        fl.append(ctx.last_counters()["flops"])
-    ms = float(np.mean(kms))
-    out.update({"value": round(total / ms * 1e3, 1), "ms_per_step": round(ms, 3),
+    learned.load(ctx, learned.saved_weights("mixed_step_mlp"))
+
+    def run(opts):
+        for _ in range(args.warmup):
+            track_multi(prog, starts, opts)
+        kms, fl, outs = [], [], None
+        for _ in range(args.steps):
+            outs = track_multi(prog, starts, opts)
+            kms.append(ctx.last_kernel_ms())
+            fl.append(ctx.last_counters()["flops"])
+        return float(np.mean(kms)), float(np.mean(fl)), outs
+
+    pms, pfl, pouts = run(make_options())
+    lms, lfl, louts = run(make_options(learned_step=1))
+    same = all(bool((a.code == b.code).all()) for a, b in zip(pouts, louts))
+    out.update({"value": round(total / lms * 1e3, 1), "ms_per_step": round(lms, 3),
                 "config": {"workload": "mixed: shared-factor systems n=4,6,8,12 (538812 total-degree paths) in one "
-                                       "launch; value from the kernel time (inputs copied from the host each launch)",
-                           "paths": total},
-                "roofline": roofline(float(np.mean(fl)), ms, "mixed"),
-                "solution_counts": {f"n{H.n}": counts(o.records) for (H, _), o in zip(probs, outs)}})
+                                       "launch, learned-step tracker; value from the kernel time (inputs are copied "
+                                       "from the host each launch)", "paths": total,
+                           "weights": "committed (hcgpu/weights/mixed_step_mlp.npy)"},
+                "roofline": roofline(lfl, lms, "mixed_learned"),
+                "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
+                "learned": {"speedup_vs_plain": round(pms / lms, 3)},
+                "identical_return_codes_learned_vs_plain": same,
+                "solution_counts": {f"n{H.n}": counts(o.records) for (H, _), o in zip(probs, louts)}})
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        import oracle  # CPU baseline only
+        cpu_s, sample = 0.0, []
+        for (H, S), g in zip(probs, louts):
+            m = min(len(S), 2048)
+            t0 = time.perf_counter()
+            o = oracle.track_batch(H, S[:m], make_options(), nthreads=args.cpu_threads)
+            dt = time.perf_counter() - t0
+            cpu_s += dt * len(S) / m
+            sample.append(f"n={H.n}: {m} of {len(S)} paths in {dt:.1f} s")
+            if not (o.code == g.code[:m]).all():
+                out.setdefault("parity_vs_oracle_sample", {})[f"n{H.n}"] = "return codes differ"
+        out["cpu_baseline"] = {"value": round(total / cpu_s, 1), "unit": "paths/s", "cores": args.cpu_threads,
+                               "kind": "port", "sample": "oracle, per size, extrapolated to the full batch: "
+                                                         + "; ".join(sample)}
+        out["gpu_vs_cpu"] = round(out["value"] / (total / cpu_s), 2)
 
 
 def main():
