@@ -425,3 +425,38 @@ def test_valuation_example_3():
     assert code == "success"
     np.testing.assert_allclose(vx, [-1 / 6, -2 / 6], atol=tf ** (1 / 6))
     np.testing.assert_allclose(vt, [-1 / 6, -2 / 6], atol=tf ** (1 / 6))
+
+
+def test_tape_emission_orders_are_equivalent():
+    """The tape producer may emit the polynomials one after the other, interleaved over shared
+    monomials, row by row, or row by row with per-row monomials (whichever keeps fewer values
+    live); every order computes each output with the same operations, so evaluation results
+    are bit for bit the same."""
+    from hcgpu import tape
+    from hcgpu.configs import cfg2_problem
+    from hcgpu.homotopies import affine_parameter_lift
+    F2, _, _ = cfg2_problem(1)
+    for S in [dense_system(5, 3, seed=4), affine_parameter_lift(F2)[0]]:
+        variants = []
+        orig = tape._best
+        tape._best = lambda makers: (variants.extend(m() for m in makers), variants[0])[1]
+        try:
+            tape.jacobian_tape(S)
+            tape.eval_tape(S)
+        finally:
+            tape._best = orig
+        assert len(variants) == 6
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal(S.nvars) + 1j * rng.standard_normal(S.nvars)
+        p = rng.standard_normal(S.nparams) + 1j * rng.standard_normal(S.nparams)
+        ref_u, ref_U = oracle.sys_eval(variants[0], x, p)
+        for t in variants[1:4]:
+            u, U = oracle.sys_eval(t, x, p)
+            assert np.array_equal(u, ref_u) and np.array_equal(U, ref_U)
+        ev = [oracle.sys_eval(t, x, p)[0] for t in variants[4:]]
+        assert np.array_equal(ev[0], ev[1]) and np.array_equal(ev[0], ref_u)
+        tx = rng.standard_normal((3, S.nvars)) + 1j * rng.standard_normal((3, S.nvars))
+        dp = rng.standard_normal(S.nparams) + 1j * rng.standard_normal(S.nparams)
+        t0 = oracle.sys_taylor(variants[4], 3, tx, p if S.nparams else None, dp if S.nparams else None)
+        t1 = oracle.sys_taylor(variants[5], 3, tx, p if S.nparams else None, dp if S.nparams else None)
+        assert np.array_equal(t0, t1)
