@@ -40,12 +40,12 @@ def measured_traffic(workload: str):
     return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0)
 
 
-def roofline(flops, kernel_ms, workload):
+def roofline(flops, kernel_ms, workload, kernel="hcg_track_kernel"):
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     return {"bound": "mfma", "bound_detail": "FP64 compute roof (MI355X FP64 vector = matrix peak 78.6 TFLOP/s)",
             "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": measured_traffic(workload),
-            "kernel": "hcg_track_kernel", "kernel_ms": round(kernel_ms, 3), "flops_per_launch": flops}
+            "kernel": kernel, "kernel_ms": round(kernel_ms, 3), "flops_per_launch": flops}
 
 
 class DeviceBatch:
@@ -280,7 +280,7 @@ def bench_mixed(args, out):
                                        "launch, learned-step tracker; value from the kernel time (inputs are copied "
                                        "from the host each launch)", "paths": total,
                            "weights": "committed (hcgpu/weights/mixed_step_mlp.npy)"},
-                "roofline": roofline(lfl, lms, "mixed_learned"),
+                "roofline": roofline(lfl, lms, "mixed_learned", kernel="hcg_track_multi_kernel"),
                 "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
                 "learned": {"speedup_vs_plain": round(pms / lms, 3)},
                 "identical_return_codes_learned_vs_plain": same,

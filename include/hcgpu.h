@@ -202,8 +202,6 @@ int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batc
                            const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
                            double* last_points_dev, double* valuations_dev);
 
-/* Time of the last hcg_track_batch's tracker kernel (ms, HIP events on its stream),
- * and the device-side counters of that launch. */
 /* Several homotopies (1..8, any sizes and kinds) compiled into one program: one
  * persistent launch tracks the paths of all of them from a single work queue.  This is the
  * batch form of calling threaded_solve (src/solve.jl:628-709) once per system; there is
@@ -230,6 +228,8 @@ int hcg_compile_ex(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t mul
 int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches, const hcg_options* opts,
                     hcg_path_result* results, double* solutions, double* last_points, double* valuations);
 
+/* Time of the last launch's tracker kernel (ms, HIP events on its stream), and the
+ * device-side counters of that launch. */
 int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps,
                           int64_t* total_newton_iters);
 /* Device counters of the last launch: [0] tracker steps, [1] Newton iterations,
