@@ -164,66 +164,149 @@ __device__ __noinline__ void H_taylor(Lane& L, int K, cplx t, cplx* u) {
 }
 
 // ----------------------------- linear algebra -----------------------------
-__device__ __forceinline__ void ws_updated(Lane& L) {
-    L.factorized = 0; L.scaled = 0;
-    for (int k = 0; k < NN * NN; k++) L.LU[k] = L.A[k];
-}
-// lu!, src/linear_algebra.jl:130-184
-// Loops over the compile-time size are unrolled for small systems, so that the loads of a
-// column are issued together (the matrices live in the lane's private memory).
+// The matrix workspace holds the Jacobian A and its LU factors.  The factorization reads A
+// (row-scaled by rs when `scaled` is set) and writes LU: no separate copy of A into LU.
+__device__ __forceinline__ void ws_updated(Lane& L) { L.factorized = 0; L.scaled = 0; }
+
 #if NN <= 12
+// Loops over the compile-time size are unrolled for small systems, so that vectors and
+// matrix columns are held in registers with compile-time indices.
 #define HCG_UNROLL _Pragma("unroll")
+// exchange v[k] and v[p] for a run-time p >= k without dynamic register indexing
+__device__ __forceinline__ void reg_swap(cplx (&v)[NN], int k, int p) {
+    cplx t = v[k], nk = t;
+    HCG_UNROLL
+    for (int m = k + 1; m < NN; m++) {
+        bool s = p == m;
+        cplx vm = v[m];
+        nk = mkc(s ? vm.re : nk.re, s ? vm.im : nk.im);  // component selects keep v in registers
+        v[m] = mkc(s ? t.re : vm.re, s ? t.im : vm.im);
+    }
+    v[k] = nk;
+}
+// lu!, src/linear_algebra.jl:130-184, in left-looking (column by column) order.  Every entry
+// receives the same updates with the same operands in the same order as in the reference's
+// right-looking loop: a column's row interchanges are applied before its updates, and the
+// L part of earlier columns is interchanged in memory when a pivot is chosen.  So the
+// factors and pivots are identical, while only one column is live in registers.
+__device__ __noinline__ void lu_factor(Lane& L) {
+    const cplx* A = L.A;
+    cplx* F = L.LU;
+    L.flops += FL_LU;
+    const bool sc = L.scaled;
+    double r[NN];
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) r[i] = sc ? L.rs[i] : 1.0;
+    int piv[NN];
+    HCG_UNROLL
+    for (int j = 0; j < NN; j++) {
+        cplx c[NN];
+        HCG_UNROLL
+        for (int i = 0; i < NN; i++) { cplx a = A[IDX(i, j)]; c[i] = sc ? c_scale(r[i], a) : a; }
+        HCG_UNROLL
+        for (int k = 0; k < j; k++) reg_swap(c, k, piv[k]);
+        HCG_UNROLL
+        for (int k = 0; k < j; k++) {
+            cplx ukj = c[k];
+            HCG_UNROLL
+            for (int i = k + 1; i < NN; i++) c[i] = c_submul(F[IDX(i, k)], ukj, c[i]);
+        }
+        int kp = j;
+        double amax = c_abs2(c[j]);
+        HCG_UNROLL
+        for (int i = j + 1; i < NN; i++) {
+            double absi = c_abs2(c[i]);
+            if (absi > amax) { kp = i; amax = absi; }
+        }
+        piv[j] = kp;
+        L.ipiv[j] = kp;
+        reg_swap(c, j, kp);
+        if (kp != j)
+            for (int k = 0; k < j; k++) { cplx t = F[IDX(j, k)]; F[IDX(j, k)] = F[IDX(kp, k)]; F[IDX(kp, k)] = t; }
+        if (amax != 0.0) {
+            cplx d = c[j];
+            HCG_UNROLL
+            for (int i = j + 1; i < NN; i++) c[i] = c_div(c[i], d);
+        }
+        HCG_UNROLL
+        for (int i = 0; i < NN; i++) F[IDX(i, j)] = c[i];
+    }
+    L.factorized = 1;
+}
+// lu_ldiv!, src/linear_algebra.jl:268-316: x = LU \ (P (rs .* b)) (rs == nullptr: no scaling);
+// x may alias b.  The vector stays in registers.
+__device__ __noinline__ void lu_solve_rs(Lane& L, cplx* x, const cplx* b, const double* rs) {
+    const cplx* F = L.LU;
+    L.flops += FL_SOLVE;
+    cplx v[NN];
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) v[i] = rs ? c_scale(rs[i], b[i]) : b[i];
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) reg_swap(v, i, L.ipiv[i]);
+    HCG_UNROLL
+    for (int j = 0; j < NN; j++) {
+        cplx xj = v[j];
+        HCG_UNROLL
+        for (int i = j + 1; i < NN; i++) v[i] = c_submul(F[IDX(i, j)], xj, v[i]);
+    }
+    HCG_UNROLL
+    for (int j = NN - 1; j >= 0; j--) {
+        cplx xj = c_div(v[j], F[IDX(j, j)]);
+        v[j] = xj;
+        HCG_UNROLL
+        for (int i = 0; i < j; i++) v[i] = c_submul(F[IDX(i, j)], xj, v[i]);
+    }
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) x[i] = v[i];
+}
 #else
 #define HCG_UNROLL
-#endif
+// lu!, src/linear_algebra.jl:130-184 (right-looking, in place on LU = diag(rs) A)
 __device__ __noinline__ void lu_factor(Lane& L) {
-    cplx* A = L.LU;
+    cplx* F = L.LU;
     L.flops += FL_LU;
-    HCG_UNROLL
+    if (L.scaled) for (int k = 0; k < NN * NN; k++) F[k] = c_scale(L.rs[k % NN], L.A[k]);
+    else for (int k = 0; k < NN * NN; k++) F[k] = L.A[k];
     for (int k = 0; k < NN; k++) {
         int kp = k;
-        double amax = c_abs2(A[IDX(k, k)]);
-        HCG_UNROLL
+        double amax = c_abs2(F[IDX(k, k)]);
         for (int i = k + 1; i < NN; i++) {
-            double absi = c_abs2(A[IDX(i, k)]);
+            double absi = c_abs2(F[IDX(i, k)]);
             if (absi > amax) { kp = i; amax = absi; }
         }
         L.ipiv[k] = kp;
         if (amax != 0.0) {
             if (k != kp)
-                for (int i = 0; i < NN; i++) { cplx t = A[IDX(k, i)]; A[IDX(k, i)] = A[IDX(kp, i)]; A[IDX(kp, i)] = t; }
-            cplx Akk = A[IDX(k, k)];
-            HCG_UNROLL
-            for (int i = k + 1; i < NN; i++) A[IDX(i, k)] = c_div(A[IDX(i, k)], Akk);
+                for (int i = 0; i < NN; i++) { cplx t = F[IDX(k, i)]; F[IDX(k, i)] = F[IDX(kp, i)]; F[IDX(kp, i)] = t; }
+            cplx Fkk = F[IDX(k, k)];
+            for (int i = k + 1; i < NN; i++) F[IDX(i, k)] = c_div(F[IDX(i, k)], Fkk);
         }
-        HCG_UNROLL
         for (int j = k + 1; j < NN; j++) {
-            cplx Akj = A[IDX(k, j)];
-            HCG_UNROLL
-            for (int i = k + 1; i < NN; i++) A[IDX(i, j)] = c_submul(A[IDX(i, k)], Akj, A[IDX(i, j)]);
+            cplx Fkj = F[IDX(k, j)];
+            for (int i = k + 1; i < NN; i++) F[IDX(i, j)] = c_submul(F[IDX(i, k)], Fkj, F[IDX(i, j)]);
         }
     }
     L.factorized = 1;
 }
-// lu_ldiv!, src/linear_algebra.jl:268-316 (in place)
-__device__ __noinline__ void lu_solve(Lane& L, cplx* x) {
-    const cplx* A = L.LU;
+// lu_ldiv!, src/linear_algebra.jl:268-316: x = LU \ (P (rs .* b)) (rs == nullptr: no scaling)
+__device__ __noinline__ void lu_solve_rs(Lane& L, cplx* x, const cplx* b, const double* rs) {
+    const cplx* F = L.LU;
     L.flops += FL_SOLVE;
+    if (rs) for (int i = 0; i < NN; i++) x[i] = c_scale(rs[i], b[i]);
+    else if (x != b) for (int i = 0; i < NN; i++) x[i] = b[i];
     for (int i = 0; i < NN; i++) { int p = L.ipiv[i]; if (i != p) { cplx t = x[i]; x[i] = x[p]; x[p] = t; } }
-    HCG_UNROLL
     for (int j = 0; j < NN; j++) {
         cplx xj = x[j];
-        HCG_UNROLL
-        for (int i = j + 1; i < NN; i++) x[i] = c_submul(A[IDX(i, j)], xj, x[i]);
+        for (int i = j + 1; i < NN; i++) x[i] = c_submul(F[IDX(i, j)], xj, x[i]);
     }
-    HCG_UNROLL
     for (int j = NN - 1; j >= 0; j--) {
-        cplx xj = c_div(x[j], A[IDX(j, j)]);
+        cplx xj = c_div(x[j], F[IDX(j, j)]);
         x[j] = xj;
-        HCG_UNROLL
-        for (int i = 0; i < j; i++) x[i] = c_submul(A[IDX(i, j)], xj, x[i]);
+        for (int i = 0; i < j; i++) x[i] = c_submul(F[IDX(i, j)], xj, x[i]);
     }
 }
+#endif
+__device__ __forceinline__ void lu_solve(Lane& L, cplx* x) { lu_solve_rs(L, x, x, nullptr); }
 // lu_ldiv_adj!, src/linear_algebra.jl:318-354
 __device__ __noinline__ void lu_solve_adj(Lane& L, cplx* x) {
     const cplx* A = L.LU;
@@ -244,9 +327,7 @@ __device__ __noinline__ void lu_solve_adj(Lane& L, cplx* x) {
 __device__ __noinline__ void ws_ldiv(Lane& L, cplx* x, const cplx* b) {
     if (NN == 1) { x[0] = c_div(b[0], L.A[0]); return; }
     if (!L.factorized) lu_factor(L);
-    if (L.scaled) for (int i = 0; i < NN; i++) x[i] = c_scale(L.rs[i], b[i]);
-    else if (x != b) for (int i = 0; i < NN; i++) x[i] = b[i];
-    lu_solve(L, x);
+    lu_solve_rs(L, x, b, L.scaled ? L.rs : nullptr);
 }
 // skeel_row_scaling!, src/linear_algebra.jl:432-459
 __device__ __noinline__ void skeel_row_scaling(double* d, const cplx* A, const double* c, double threshold) {
@@ -328,8 +409,7 @@ __device__ __noinline__ double iterative_refinement(Lane& L, cplx* x, const cplx
 // Jacobian ldiv! with WeightedNorm (src/linear_algebra.jl:846-862)
 __device__ __noinline__ void jac_ldiv_w(Lane& L, cplx* x, const cplx* b) {
     if (!L.factorized) {
-        skeel_row_scaling(L.rs, L.A, L.w, -30.0);
-        for (int j = 0; j < NN; j++) for (int i = 0; i < NN; i++) L.LU[IDX(i, j)] = c_scale(L.rs[i], L.LU[IDX(i, j)]);
+        skeel_row_scaling(L.rs, L.A, L.w, -30.0);  // lu_factor applies it to the rows of A
         L.scaled = 1;
     }
     ws_ldiv(L, x, b);
