@@ -15,6 +15,7 @@
 
 #include <array>
 #include <climits>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <sstream>
@@ -76,12 +77,32 @@ bool validate_tape(const hcg_tape* tp, int n, std::string* err) {
         if (tp->u_assign[2 * i] < 1 || tp->u_assign[2 * i] > n || tp->u_assign[2 * i + 1] < 1 || tp->u_assign[2 * i + 1] > tp->tape_space) { *err = "bad u assignment"; return false; }
     for (int i = 0; i < tp->nU; i++)
         if (tp->U_assign[2 * i] < 1 || tp->U_assign[2 * i] > n * n || tp->U_assign[2 * i + 1] < 1 || tp->U_assign[2 * i + 1] > tp->tape_space) { *err = "bad U assignment"; return false; }
+    std::vector<char> seen(n + (size_t)n * n, 0);
+    for (int i = 0; i < tp->nu; i++)
+        if (seen[tp->u_assign[2 * i] - 1]++) { *err = "u entry assigned twice"; return false; }
+    for (int i = 0; i < tp->nU; i++)
+        if (seen[n + tp->U_assign[2 * i] - 1]++) { *err = "U entry assigned twice"; return false; }
     return true;
 }
 
 enum ParamMode { PM_NONE, PM_FIXED, PM_DYNAMIC };
 
 // ---------------------------- ComplexF64 / DD eval ----------------------------
+// How an output is written: store(kind 'u' or 'U', index, value expression) -> statement.
+// The default writes the value; the fused straight-line Jacobian combines it with the
+// other system's output instead.
+typedef std::function<std::string(char, int, const std::string&)> StoreFn;
+
+static std::string plain_store(char kind, int idx, const std::string& v) {
+    return std::string(1, kind) + "[" + std::to_string(idx) + "] = " + v + ";";
+}
+
+// Straight-line body of one tape (no signature): inputs x (and pp, pq, pt when pm is
+// PM_DYNAMIC), outputs through `store`; local names carry `pfx` so that two bodies can
+// share one function.  Returns the counted FP64 flops.
+static long gen_eval_body(std::ostringstream& os, const hcg_tape* tp, bool jac, bool dd, int n, ParamMode pm,
+                          const double* fixed, const std::string& pfx, const StoreFn& store);
+
 static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, bool jac, bool dd, int n,
                      ParamMode pm, const double* fixed, long* flops) {
     const char* T = dd ? "cdd" : "cplx";
@@ -90,6 +111,13 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     os << (dd ? "cdd" : "cplx") << "* __restrict__ u";
     if (jac) os << ", cplx* __restrict__ U";
     os << ") {\n";
+    *flops = gen_eval_body(os, tp, jac, dd, n, pm, fixed, "", plain_store);
+    os << "}\n";
+}
+
+static long gen_eval_body(std::ostringstream& os, const hcg_tape* tp, bool jac, bool dd, int n, ParamMode pm,
+                          const double* fixed, const std::string& pfx, const StoreFn& store) {
+    const char* T = dd ? "cdd" : "cplx";
     std::vector<std::string> nm(tp->tape_space + 1);
     auto wrap = [&](const std::string& c) { return dd ? "cdd_from(" + c + ")" : c; };
     for (int k = 0; k < tp->nconst; k++)
@@ -98,13 +126,13 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     // ranges (and the register pressure of long straight-line bodies) stay short
     std::vector<std::string> lazy_def(tp->tape_space + 1);
     for (int k = 0; k < tp->nparams; k++) {
-        std::string v = "P" + std::to_string(k);
+        std::string v = pfx + "P" + std::to_string(k);
         if (pm == PM_FIXED) lazy_def[tp->params_first + k] = "  const " + std::string(T) + " " + v + " = " + wrap(clit(fixed[2 * k], fixed[2 * k + 1])) + ";\n";
         else lazy_def[tp->params_first + k] = "  const " + std::string(T) + " " + v + " = " + wrap("par_lerp(pt, pp, pq, " + std::to_string(k) + ")") + ";\n";
         nm[tp->params_first + k] = v;
     }
     for (int k = 0; k < tp->nvars; k++) {
-        std::string v = "X" + std::to_string(k);
+        std::string v = pfx + "X" + std::to_string(k);
         lazy_def[tp->vars_first + k] = "  const " + std::string(T) + " " + v + " = x[" + std::to_string(k) + "];\n";
         nm[tp->vars_first + k] = v;
     }
@@ -122,27 +150,26 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
         if (I[4] == OP_STOP) break;
         last_write[I[5]] = i;
     }
-    std::vector<std::vector<std::string>> stores_after(tp->ninstr + 1);
+    struct OutStore { char kind; int idx, slot; };
+    std::vector<std::vector<OutStore>> stores_after(tp->ninstr + 1);
     std::vector<int> uset(n, 0), Uset(jac ? n * n : 0, 0);
     for (int i = 0; i < tp->nu; i++) {
         int row = tp->u_assign[2 * i] - 1, slot = tp->u_assign[2 * i + 1];
         int at = last_write[slot] >= 0 ? last_write[slot] : tp->ninstr;
-        stores_after[at].push_back("u[" + std::to_string(row) + "]#" + std::to_string(slot));
+        stores_after[at].push_back({'u', row, slot});
         uset[row] = 1;
     }
     if (jac)
         for (int i = 0; i < tp->nU; i++) {
             int l = tp->U_assign[2 * i] - 1, slot = tp->U_assign[2 * i + 1];
             int at = last_write[slot] >= 0 ? last_write[slot] : tp->ninstr;
-            stores_after[at].push_back("U[" + std::to_string(l) + "]#" + std::to_string(slot));
+            stores_after[at].push_back({'U', l, slot});
             Uset[l] = 1;
         }
     auto flush_stores = [&](int at) {
         for (auto& st : stores_after[at]) {
-            size_t h = st.find('#');
-            int slot = std::stoi(st.substr(h + 1));
-            need(slot);
-            os << "  " << st.substr(0, h) << " = " << nm[slot] << ";\n";
+            need(st.slot);
+            os << "  " << store(st.kind, st.idx, nm[st.slot]) << "\n";
         }
     };
     for (int i = 0; i < tp->ninstr; i++) {
@@ -153,7 +180,7 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
             if (!(op == OP_POW_INT && q == 1)) need(I[q]);
         std::string a = nm[I[0]], b = arity(op) > 1 && op != OP_POW_INT ? nm[I[1]] : "", c = arity(op) > 2 ? nm[I[2]] : "",
                     d = arity(op) > 3 ? nm[I[3]] : "";
-        std::string out = "T" + std::to_string(i);
+        std::string out = pfx + "T" + std::to_string(i);
         std::string e;
         switch (op) {
         case OP_CB: e = std::string(pre) + "cb(" + a + ")"; f += dm * 2; break;
@@ -184,12 +211,52 @@ static void gen_eval(std::ostringstream& os, const hcg_tape* tp, const std::stri
     }
     flush_stores(tp->ninstr);
     for (int i = 0; i < n; i++)
-        if (!uset[i]) os << "  u[" << i << "] = " << (dd ? "cdd_from(mkc(0.0, 0.0))" : "mkc(0.0, 0.0)") << ";\n";
+        if (!uset[i]) os << "  " << store('u', i, dd ? "cdd_from(mkc(0.0, 0.0))" : "mkc(0.0, 0.0)") << "\n";
     if (jac)
         for (int i = 0; i < n * n; i++)
-            if (!Uset[i]) os << "  U[" << i << "] = mkc(0.0, 0.0);\n";
+            if (!Uset[i]) os << "  " << store('U', i, "mkc(0.0, 0.0)") << "\n";
+    return f;
+}
+
+// Output values of a tape that are compile-time constants: the literal for each output
+// (u: 0..n-1, U: n..n+n*n-1) whose slot is a tape constant or that is never assigned,
+// "" for outputs the tape computes.
+static std::vector<std::string> constant_outputs(const hcg_tape* tp, int n) {
+    std::vector<std::string> c(n + n * n, "mkc(0.0, 0.0)");
+    auto lit = [&](int slot) -> std::string {
+        if (slot >= 1 && slot <= tp->nconst) return clit(tp->constants[2 * (slot - 1)], tp->constants[2 * (slot - 1) + 1]);
+        return "";
+    };
+    for (int i = 0; i < tp->nu; i++) c[tp->u_assign[2 * i] - 1] = lit(tp->u_assign[2 * i + 1]);
+    for (int i = 0; i < tp->nU; i++) c[n + tp->U_assign[2 * i] - 1] = lit(tp->U_assign[2 * i + 1]);
+    return c;
+}
+
+// Straight-line homotopy H = ts G + tt F (ts = γ t, tt = 1 - t): value and Jacobian in one
+// function, each entry combined exactly as c_muladd(ts, G_l, c_mul(tt, F_l)).  F's body runs
+// first; entries whose G value is a compile-time constant are combined where F stores them,
+// the others are stored raw and combined where G's body stores its value, so no second
+// Jacobian is written to memory.
+static long gen_straight_eval_jac(std::ostringstream& os, const hcg_system& G, const hcg_system& F, int n,
+                                  ParamMode pmG, ParamMode pmF) {
+    os << "__device__ __noinline__ void sysH_eval_jac(const cplx* __restrict__ x, cplx ts, cplx tt, "
+          "cplx* __restrict__ u, cplx* __restrict__ U) {\n";
+    const std::vector<std::string> gc = constant_outputs(G.eval_jac, n);
+    auto pos = [n](char kind, int idx) { return kind == 'u' ? idx : n + idx; };
+    long f = gen_eval_body(os, F.eval_jac, true, false, n, pmF, F.fixed_params, "f",
+                           [&](char kind, int idx, const std::string& v) {
+                               const std::string& g = gc[pos(kind, idx)];
+                               if (g.empty()) return plain_store(kind, idx, v);
+                               return plain_store(kind, idx, "c_muladd(ts, " + g + ", c_mul(tt, " + v + "))");
+                           });
+    f += gen_eval_body(os, G.eval_jac, true, false, n, pmG, G.fixed_params, "g",
+                       [&](char kind, int idx, const std::string& v) -> std::string {
+                           if (!gc[pos(kind, idx)].empty()) return "";
+                           std::string dst = std::string(1, kind) + "[" + std::to_string(idx) + "]";
+                           return dst + " = c_muladd(ts, " + v + ", c_mul(tt, " + dst + "));";
+                       });
     os << "}\n";
-    *flops = f;
+    return f;
 }
 
 // ------------------------ double-double tape interpreter ------------------------
@@ -344,12 +411,30 @@ struct SVal {
     std::vector<std::string> c; // coefficient expressions (valid for 0..deg)
 };
 
+// How a row's Taylor coefficients of orders K-1 and K are written: (row, vKm1, vK) -> statements.
+typedef std::function<std::string(int, const std::string&, const std::string&)> TaylorStoreFn;
+
+static long gen_taylor_body(std::ostringstream& os, const hcg_tape* tp, int K, int n, ParamMode pm, const double* fixed,
+                            const std::string& pfx, const TaylorStoreFn& store);
+
 static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, int K, int n, ParamMode pm,
                        const double* fixed, long* flops) {
     os << "__device__ __noinline__ void " << fname << "(";
     for (int j = 0; j < K; j++) os << "const cplx* __restrict__ t" << j << ", ";
     if (pm == PM_DYNAMIC) os << "const double* __restrict__ pp, const double* __restrict__ pq, cplx pt, ";
-    os << "cplx* __restrict__ oKm1, cplx* __restrict__ oK) {\n";
+    // the parameter homotopy needs order K only (and its tracker tail-calls this function)
+    const bool km1 = pm != PM_DYNAMIC;
+    if (km1) os << "cplx* __restrict__ oKm1, ";
+    os << "cplx* __restrict__ oK) {\n";
+    *flops = gen_taylor_body(os, tp, K, n, pm, fixed, "", [km1](int row, const std::string& a, const std::string& b) {
+        std::string r = km1 ? "oKm1[" + std::to_string(row) + "] = " + a + "; " : std::string();
+        return r + "oK[" + std::to_string(row) + "] = " + b + ";";
+    });
+    os << "}\n";
+}
+
+static long gen_taylor_body(std::ostringstream& os, const hcg_tape* tp, int K, int n, ParamMode pm, const double* fixed,
+                            const std::string& pfx, const TaylorStoreFn& store) {
     std::vector<SVal> v(tp->tape_space + 1);
     auto zero = [&]() { SVal s; s.deg = -1; return s; };
     for (auto& s : v) s = zero();
@@ -361,14 +446,14 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
     std::vector<std::string> lazy_def(tp->tape_space + 1);
     for (int k = 0; k < tp->nparams; k++) {
         SVal s;
-        std::string nm0 = "P" + std::to_string(k);
+        std::string nm0 = pfx + "P" + std::to_string(k);
         std::string& d = lazy_def[tp->params_first + k];
         if (pm == PM_FIXED) {
             d = "  const cplx " + nm0 + " = " + clit(fixed[2 * k], fixed[2 * k + 1]) + ";\n";
             s.deg = 0; s.c.push_back(nm0);
         } else {
             d = "  const cplx " + nm0 + " = par_lerp(pt, pp, pq, " + std::to_string(k) + ");\n";
-            std::string nm1 = "Q" + std::to_string(k);
+            std::string nm1 = pfx + "Q" + std::to_string(k);
             if (K >= 1) d += "  const cplx " + nm1 + " = par_diff(pp, pq, " + std::to_string(k) + ");\n";
             s.deg = K >= 1 ? 1 : 0; s.c.push_back(nm0);
             if (K >= 1) s.c.push_back(nm1);
@@ -380,7 +465,7 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         s.deg = K - 1;
         std::string& d = lazy_def[tp->vars_first + k];
         for (int j = 0; j < K; j++) {
-            std::string nm = "X" + std::to_string(k) + "_" + std::to_string(j);
+            std::string nm = pfx + "X" + std::to_string(k) + "_" + std::to_string(j);
             d += "  const cplx " + nm + " = t" + std::to_string(j) + "[" + std::to_string(k) + "];\n";
             s.c.push_back(nm);
         }
@@ -417,8 +502,8 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         for (auto& rs : stores_after[at]) {
             need(rs.second);
             const SVal& sv = v[rs.second];
-            os << "  oKm1[" << rs.first << "] = " << (has(sv, K - 1) ? sv.c[K - 1] : "mkc(0.0, 0.0)") << ";\n";
-            os << "  oK[" << rs.first << "] = " << (has(sv, K) ? sv.c[K] : "mkc(0.0, 0.0)") << ";\n";
+            os << "  " << store(rs.first, has(sv, K - 1) ? sv.c[K - 1] : "mkc(0.0, 0.0)", has(sv, K) ? sv.c[K] : "mkc(0.0, 0.0)")
+               << "\n";
         }
     };
     for (int i = 0; i < tp->ninstr; i++) {
@@ -427,7 +512,7 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
         if (op == OP_STOP) break;
         for (int q = 0; q < arity(op); q++)
             if (!(op == OP_POW_INT && q == 1)) need(I[q]);
-        std::string base = "S" + std::to_string(i);
+        std::string base = pfx + "S" + std::to_string(i);
         const SVal& a = v[I[0]];
         SVal r;
         auto emit = [&](int k, const std::string& e) {
@@ -543,19 +628,55 @@ static void gen_taylor(std::ostringstream& os, const hcg_tape* tp, const std::st
     }
     flush_stores(tp->ninstr);
     for (int i = 0; i < n; i++)
-        if (!uset[i]) os << "  oKm1[" << i << "] = mkc(0.0, 0.0);\n  oK[" << i << "] = mkc(0.0, 0.0);\n";
-    os << "}\n";
-    *flops = f;
+        if (!uset[i]) os << "  " << store(i, "mkc(0.0, 0.0)", "mkc(0.0, 0.0)") << "\n";
+    return f;
 }
 
+// Straight-line homotopy, Taylor coefficient of order K of H = γ t G + (1 - t) F along the
+// path (x(t) = Σ t_j τ^j):  order 1 (values):  u = γ G(x) - F(x);  order K >= 2:
+// u = γ (G_{K-1} + t G_K) + ((1 - t) F_K - F_{K-1}).  Both bodies in one function: F's part
+// is stored first and G's part added where G's body stores its row (same operations and
+// order as combining two separately stored results).
+static long gen_straight_taylor(std::ostringstream& os, const hcg_system& G, const hcg_system& F, int K, int n,
+                                ParamMode pmG, ParamMode pmF) {
+    long f = 0;
+    os << "__device__ __noinline__ void sysH_taylor" << K << "(";
+    if (K == 1) os << "const cplx* __restrict__ x, ";
+    else
+        for (int j = 0; j < K; j++) os << "const cplx* __restrict__ t" << j << ", ";
+    os << "cplx g, cplx t, cplx omt, cplx* __restrict__ u) {\n";
+    if (K == 1) {
+        f += gen_eval_body(os, F.eval, false, false, n, pmF, F.fixed_params, "f",
+                           [](char, int i, const std::string& v) { return plain_store('u', i, v); });
+        f += gen_eval_body(os, G.eval, false, false, n, pmG, G.fixed_params, "g", [](char, int i, const std::string& v) {
+            std::string d = "u[" + std::to_string(i) + "]";
+            return d + " = c_sub(c_mul(g, " + v + "), " + d + ");";
+        });
+    } else {
+        f += gen_taylor_body(os, F.eval, K, n, pmF, F.fixed_params, "f", [](int i, const std::string& a, const std::string& b) {
+            return "u[" + std::to_string(i) + "] = c_sub(c_mul(omt, " + b + "), " + a + ");";
+        });
+        f += gen_taylor_body(os, G.eval, K, n, pmG, G.fixed_params, "g", [](int i, const std::string& a, const std::string& b) {
+            std::string d = "u[" + std::to_string(i) + "]";
+            return d + " = c_add(c_mul(g, c_add(" + a + ", c_mul(t, " + b + "))), " + d + ");";
+        });
+    }
+    os << "}\n";
+    return f;
+}
+
+// Functions of one system.  A straight-line homotopy's Jacobian and Taylor coefficients come
+// from the fused sysH_* functions instead (gen_straight_*), so only its values are generated
+// per system.
 static void gen_system(std::ostringstream& os, const hcg_system& s, const std::string& prefix, int n, ParamMode pm,
                        bool straight, Flops* fl) {
     const double* fixed = s.fixed_params;
     long f;
     gen_eval(os, s.eval, prefix + "_eval", false, false, n, pm, fixed, &f); fl->eval += f;
-    gen_eval(os, s.eval_jac, prefix + "_eval_jac", true, false, n, pm, fixed, &f); fl->eval_jac += f;
     gen_dd_interp(os, s.eval, prefix + "_eval_dd", n, pm, fixed, &f); fl->eval_dd += f;
-    for (int K = straight ? 2 : 1; K <= 3; K++) {
+    if (straight) return;
+    gen_eval(os, s.eval_jac, prefix + "_eval_jac", true, false, n, pm, fixed, &f); fl->eval_jac += f;
+    for (int K = 1; K <= 3; K++) {
         gen_taylor(os, s.eval, prefix + "_taylor" + std::to_string(K), K, n, pm, fixed, &f);
         fl->taylor[K] += f;
     }
@@ -583,8 +704,12 @@ static bool emit_system(std::ostringstream& os, const hcg_homotopy_desc* h, Flop
     os << "#define HCG_NPAR " << (h->kind == HCG_PARAMETER ? h->target.eval->nparams : 0) << "\n";
     os << "#define HCG_GAMMA_RE " << dlit(h->gamma_re) << "\n#define HCG_GAMMA_IM " << dlit(h->gamma_im) << "\n";
     if (h->kind == HCG_STRAIGHT_LINE) {
-        gen_system(os, h->start, "sysG", n, h->start.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
-        gen_system(os, h->target, "sysF", n, h->target.eval->nparams ? PM_FIXED : PM_NONE, true, fl);
+        const ParamMode pmG = h->start.eval->nparams ? PM_FIXED : PM_NONE, pmF = h->target.eval->nparams ? PM_FIXED : PM_NONE;
+        gen_system(os, h->start, "sysG", n, pmG, true, fl);
+        gen_system(os, h->target, "sysF", n, pmF, true, fl);
+        fl->eval_jac += gen_straight_eval_jac(os, h->start, h->target, n, pmG, pmF);
+        for (int K = 2; K <= 3; K++) fl->taylor[K] += gen_straight_taylor(os, h->start, h->target, K, n, pmG, pmF);
+        gen_straight_taylor(os, h->start, h->target, 1, n, pmG, pmF);  // values: counted as fl->eval
     } else {
         gen_system(os, h->target, "sysF", n, PM_DYNAMIC, false, fl);
     }
@@ -624,7 +749,7 @@ bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::st
     if (nsys < 1 || nsys > 8) { *err = "between 1 and 8 systems per program"; return false; }
     std::ostringstream os;
     os << "// generated by hcgpu codegen (multi-system program)\n";
-    os << "#include \"hcg_arith.h\"\n#include \"hcgpu.h\"\n#include \"hcg_kargs.h\"\n#include \"hcg_mlp_infer.h\"\n";
+    os << "#include \"hcg_arith.h\"\n#include \"hcgpu.h\"\n#include \"hcg_kargs.h\"\n#include \"hcg_sched.h\"\n#include \"hcg_mlp_infer.h\"\n";
     for (int s = 0; s < nsys; s++) {
         os << "namespace hcs" << s << " {\n";
         std::string e;
@@ -655,11 +780,14 @@ bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::st
     const unsigned short* mw = stage_mlp(m.sys[0].mlp, m.sys[0].opts, mlp_lds);
     hcg_lane_union U;
     LaneTotals T = {0, 0, 0, 0.0};
+    const int refill = refill_threshold(m.sys[0].opts);
     int phase = 0, k = -1, s = 0;
     while (true) {
         int ended = 0;
+        const unsigned long long fetch = wave_refill_mask(phase == 0, refill);
         if (phase == 0) {
-            int kg = atomicAdd(m.next_path, 1);
+            if (!fetch) continue;
+            int kg = wave_fetch(fetch, m.next_path);
             if (kg >= m.total) break;
             s = 0;
             while (s + 1 < m.nsys && kg >= m.offset[s + 1]) s++;

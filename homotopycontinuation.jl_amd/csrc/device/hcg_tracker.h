@@ -3,8 +3,8 @@
 // Compiled by hipRTC together with the generated system functions of one homotopy
 // (HCG_N, HCG_KIND, sysG_* / sysF_*).  Each lane repeatedly takes a path index from a
 // device work queue (atomic counter), tracks it with one tracker step per loop
-// iteration, and writes its PathResult record when the path ends; a finished lane
-// immediately takes the next path (compaction of finished paths).
+// iteration, and writes its PathResult record when the path ends; finished lanes of a
+// wave take their next paths together (hcg_sched.h).
 //
 // Algorithm (restated from the reference, same arithmetic contract as the oracle):
 //   tracker            src/tracker.jl:517-996     predictor  src/predictor.jl:124-371
@@ -15,6 +15,7 @@
 #include "hcg_arith.h"
 #include "hcgpu.h"
 #include "hcg_kargs.h"
+#include "hcg_sched.h"
 #include "hcg_mlp_infer.h"
 #include "hcg_tracker_body.h"
 
@@ -24,11 +25,14 @@ extern "C" __global__ void __launch_bounds__(64, HCG_WAVES_PER_EU) hcg_track_ker
     Lane L;
     lane_setup(L, a, mw);
     LaneTotals T = {0, 0, 0, 0.0};
-    int phase = 0;  // 0: fetch, 1: track
+    const int refill = refill_threshold(a.opts);
+    int phase = 0;  // 0: idle, 1: tracking
     int k = -1;
     while (true) {
+        const unsigned long long fetch = wave_refill_mask(phase == 0, refill);
         if (phase == 0) {
-            k = atomicAdd(a.next_path, 1);
+            if (!fetch) continue;  // wait for more idle lanes of this wave
+            k = wave_fetch(fetch, a.next_path);
             if (k >= a.npaths) break;
             if (!lane_begin(L, a, k)) continue;
             phase = 1;

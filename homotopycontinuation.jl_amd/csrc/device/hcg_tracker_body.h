@@ -122,13 +122,8 @@ __device__ __noinline__ void H_eval_dd(Lane& L, const cplx* x, cplx t, cplx* u) 
 __device__ __noinline__ void H_eval_jac(Lane& L, const cplx* x, cplx t, cplx* u, cplx* U) {
     L.flops += HCG_FL_EJ;
 #if HCG_KIND == 0
-    cplx u2[NN];
-    cplx U2[NN * NN];
-    sysG_eval_jac(x, u, U);
-    sysF_eval_jac(x, u2, U2);
-    cplx ts = c_mul(mkc(HCG_GAMMA_RE, HCG_GAMMA_IM), t), tt = mkc(1.0 - t.re, -t.im);
-    for (int i = 0; i < NN; i++) u[i] = c_muladd(ts, u[i], c_mul(tt, u2[i]));
-    for (int k = 0; k < NN * NN; k++) U[k] = c_muladd(ts, U[k], c_mul(tt, U2[k]));
+    // u = ts G + tt F entry by entry, in one generated function (no second Jacobian in memory)
+    sysH_eval_jac(x, c_mul(mkc(HCG_GAMMA_RE, HCG_GAMMA_IM), t), mkc(1.0 - t.re, -t.im), u, U);
 #else
     sysF_eval_jac(x, L.p, L.q, t, u, U);
 #endif
@@ -138,28 +133,17 @@ __device__ __noinline__ void H_eval_jac(Lane& L, const cplx* x, cplx t, cplx* u,
 __device__ __noinline__ void H_taylor(Lane& L, int K, cplx t, cplx* u) {
     L.flops += K == 1 ? HCG_FL_T1 : (K == 2 ? HCG_FL_T2 : HCG_FL_T3);
 #if HCG_KIND == 0
-    cplx g = mkc(HCG_GAMMA_RE, HCG_GAMMA_IM);
-    if (K == 1) {
-        cplx u2[NN];
-        sysG_eval(L.tx0, u);
-        sysF_eval(L.tx0, u2);
-        for (int i = 0; i < NN; i++) u[i] = c_sub(c_mul(g, u[i]), u2[i]);
-        return;
-    }
-    cplx gK1[NN], gK[NN], fK1[NN], fK[NN];
-    if (K == 2) { sysG_taylor2(L.tx0, L.tx1, gK1, gK); sysF_taylor2(L.tx0, L.tx1, fK1, fK); }
-    else { sysG_taylor3(L.tx0, L.tx1, L.tx2, gK1, gK); sysF_taylor3(L.tx0, L.tx1, L.tx2, fK1, fK); }
-    cplx omt = mkc(1.0 - t.re, -t.im);
-    for (int i = 0; i < NN; i++) {
-        cplx st = c_mul(g, c_add(gK1[i], c_mul(t, gK[i])));
-        cplx tg = c_sub(c_mul(omt, fK[i]), fK1[i]);
-        u[i] = c_add(st, tg);
-    }
+    // u = γ (G_{K-1} + t G_K) + ((1 - t) F_K - F_{K-1}) (K = 1: γ G - F), one generated
+    // function per order, tail-called
+    const cplx g = mkc(HCG_GAMMA_RE, HCG_GAMMA_IM), omt = mkc(1.0 - t.re, -t.im);
+    if (K == 1) sysH_taylor1(L.tx0, g, t, omt, u);
+    else if (K == 2) sysH_taylor2(L.tx0, L.tx1, g, t, omt, u);
+    else sysH_taylor3(L.tx0, L.tx1, L.tx2, g, t, omt, u);
 #else
-    cplx oK1[NN];
-    if (K == 1) sysF_taylor1(L.tx0, L.p, L.q, t, oK1, u);
-    else if (K == 2) sysF_taylor2(L.tx0, L.tx1, L.p, L.q, t, oK1, u);
-    else sysF_taylor3(L.tx0, L.tx1, L.tx2, L.p, L.q, t, oK1, u);
+    // tail calls (no local buffers): no register save and restore around the generated body
+    if (K == 1) sysF_taylor1(L.tx0, L.p, L.q, t, u);
+    else if (K == 2) sysF_taylor2(L.tx0, L.tx1, L.p, L.q, t, u);
+    else sysF_taylor3(L.tx0, L.tx1, L.tx2, L.p, L.q, t, u);
 #endif
 }
 
@@ -331,15 +315,24 @@ __device__ __noinline__ void ws_ldiv(Lane& L, cplx* x, const cplx* b) {
 }
 // skeel_row_scaling!, src/linear_algebra.jl:432-459
 __device__ __noinline__ void skeel_row_scaling(double* d, const cplx* A, const double* c, double threshold) {
-    for (int i = 0; i < NN; i++) d[i] = 0.0;
-    for (int j = 0; j < NN; j++) { double cj = c[j]; for (int i = 0; i < NN; i++) d[i] += c_fabs(A[IDX(i, j)]) * cj; }
-    double m = d[0];
-    for (int i = 1; i < NN; i++) m = jmax(m, d[i]);
+    double r[NN];  // row sums in registers, written to d once
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) r[i] = 0.0;
+    HCG_UNROLL
+    for (int j = 0; j < NN; j++) {
+        double cj = c[j];
+        HCG_UNROLL
+        for (int i = 0; i < NN; i++) r[i] += c_fabs(A[IDX(i, j)]) * cj;
+    }
+    double m = r[0];
+    HCG_UNROLL
+    for (int i = 1; i < NN; i++) m = jmax(m, r[i]);
     double s = threshold + m;
+    HCG_UNROLL
     for (int i = 0; i < NN; i++) {
         int e = 0;
-        if (isfinite(d[i])) frexp(d[i], &e);
-        if ((double)e < s) d[i] = 1.0; else d[i] = ldexp(1.0, -e);
+        if (isfinite(r[i])) frexp(r[i], &e);
+        d[i] = (double)e < s ? 1.0 : ldexp(1.0, -e);
     }
 }
 // weighted / plain inf-norms, src/norm.jl:150-234 (w == nullptr: InfNorm)
@@ -388,9 +381,17 @@ __device__ __noinline__ double mixed_refine(Lane& L, cplx* x, const cplx* b, con
 // fixed_precision_iterative_refinement!, src/linear_algebra.jl:553-567
 __device__ __noinline__ double fixed_refine(Lane& L, cplx* x, const cplx* b, const double* w) {
     L.flops += FL_FIXED - FL_SOLVE;
-    for (int i = 0; i < NN; i++) L.tmp[i] = mkc(0.0, 0.0);
-    for (int j = 0; j < NN; j++) { cplx xj = x[j]; for (int i = 0; i < NN; i++) L.tmp[i] = c_muladd(L.A[IDX(i, j)], xj, L.tmp[i]); }
-    for (int i = 0; i < NN; i++) L.tmp[i] = c_sub(L.tmp[i], b[i]);
+    cplx acc[NN];  // A x accumulated in registers
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) acc[i] = mkc(0.0, 0.0);
+    HCG_UNROLL
+    for (int j = 0; j < NN; j++) {
+        cplx xj = x[j];
+        HCG_UNROLL
+        for (int i = 0; i < NN; i++) acc[i] = c_muladd(L.A[IDX(i, j)], xj, acc[i]);
+    }
+    HCG_UNROLL
+    for (int i = 0; i < NN; i++) L.tmp[i] = c_sub(acc[i], b[i]);
     ws_ldiv(L, L.rdx, L.tmp);
     for (int i = 0; i < NN; i++) x[i] = c_sub(x[i], L.rdx[i]);
     return norm_inf(L.rdx, w) / norm_inf(x, w);

@@ -260,7 +260,7 @@ static int hcg_generate_impl(const hcg_homotopy_desc* h, char* buf, size_t bufsz
 }
 
 // Default register budget of the tracker kernels: waves per SIMD they are built for.
-static const int kDefaultWavesPerEU = 4;
+static const int kDefaultWavesPerEU = 2;
 
 static int build_source(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi, const hcg_build_options* bo,
                         std::string* src, std::vector<hcg::Flops>* fl) {

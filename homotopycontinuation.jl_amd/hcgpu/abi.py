@@ -81,6 +81,7 @@ class hcg_options(ctypes.Structure):
         ("scale_min", _D), ("scale_abs_min", _D), ("scale_max", _D),
         ("learned_step", _I), ("block_size", _I), ("blocks_per_cu", _I), ("log_steps", _I),
         ("learned_max_mult", _D), ("learned_min_t", _D), ("learned_inf_t", _D), ("learned_inf_logit", _D),
+        ("refill_idle", _I), ("reserved_i", _I),
     ]
 
 
@@ -112,6 +113,7 @@ def default_options(**overrides) -> hcg_options:
     o.learned_step, o.block_size, o.blocks_per_cu, o.log_steps = 0, 0, 0, 0
     o.learned_max_mult, o.learned_min_t = 4.0, 0.0
     o.learned_inf_t, o.learned_inf_logit = 0.0, 4.0
+    o.refill_idle = 0
     presets = {"default": (3.0, 0.4), "fast": (2.0, 0.75), "conservative": (4.0, 0.25)}
     params = overrides.pop("parameters", None)
     if params is not None:

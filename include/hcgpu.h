@@ -128,6 +128,10 @@ typedef struct hcg_options {
                                  learned_inf_logit) starts its endgame once t <= learned_inf_t instead
                                  of endgame_start; 0 = the flag is not used */
     double learned_inf_logit;
+    int32_t refill_idle;         /* scheduling (no counterpart in the reference): finished lanes of a
+                                    wave start their next paths together once at least this many of
+                                    the wave's lanes are idle, or all are; 0 = default 8, 1 = at once */
+    int32_t reserved_i;
 } hcg_options;
 
 /* A batch of paths for one program. */

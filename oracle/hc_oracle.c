@@ -72,6 +72,7 @@ typedef struct orc_options {
     double scale_min, scale_abs_min, scale_max;
     int32_t learned_step, block_size, blocks_per_cu, log_steps;
     double learned_max_mult, learned_min_t, learned_inf_t, learned_inf_logit;
+    int32_t refill_idle, reserved_i;  /* GPU lane scheduling only */
 } orc_options;
 typedef struct orc_batch {
     int32_t npaths, ngroups;

@@ -98,3 +98,23 @@ def test_known_answers_gpu(gpu_ctx, name):
         assert sum(1 for r in R if r.is_success()) == 5
     else:
         assert sorted(np.round(np.real([r.solution[0] for r in R])).astype(int)) == list(range(1, 13))
+
+
+@pytest.mark.parametrize("refill,bpc", [(1, 0), (5, 2), (64, 0), (1000, 1)])
+def test_lane_schedule_does_not_change_results(gpu_ctx, refill, bpc):
+    """Which lane tracks which path, and when a wave's idle lanes start their next paths
+    (refill_idle: at once, in groups, only when the whole wave is idle), changes only the
+    schedule: every path's record and endpoint are bit for bit those of the default launch,
+    and agree with the oracle."""
+    H, S = total_degree(dense_system(4, 3, seed=100), seed=101)
+    S = np.concatenate([S] * 5)  # 405 paths: several waves, each starting many paths
+    T = EndgameTracker(H, ctx=gpu_ctx)
+    T.options = make_options()
+    base = T.track_batch(S)
+    T.options = make_options(refill_idle=refill, blocks_per_cu=bpc)
+    g = T.track_batch(S)
+    assert list(g.code) == list(base.code)
+    np.testing.assert_array_equal(g.solution, base.solution)
+    assert (g.accepted_steps == base.accepted_steps).all() and (g.rejected_steps == base.rejected_steps).all()
+    sub = type(g)(g.records[:81], g.solution[:81], g.last_point[:81], g.valuation[:81])
+    assert_parity(sub, oracle.track_batch(H, S[:81], make_options()))
