@@ -171,8 +171,9 @@ __device__ __forceinline__ void reg_swap(cplx (&v)[NN], int k, int p) {
 // lu!, src/linear_algebra.jl:130-184, in left-looking (column by column) order.  Every entry
 // receives the same updates with the same operands in the same order as in the reference's
 // right-looking loop: a column's row interchanges are applied before its updates, and the
-// L part of earlier columns is interchanged in memory when a pivot is chosen.  So the
-// factors and pivots are identical, while only one column is live in registers.
+// multipliers of earlier columns are interchanged when a pivot is chosen.  So the factors
+// and pivots are identical.  The working column and the multipliers are register arrays with
+// compile-time indices (interchanges by selects); A is read once and LU written once.
 __device__ __noinline__ void lu_factor(Lane& L) {
     const cplx* A = L.A;
     cplx* F = L.LU;
@@ -182,6 +183,7 @@ __device__ __noinline__ void lu_factor(Lane& L) {
     HCG_UNROLL
     for (int i = 0; i < NN; i++) r[i] = sc ? L.rs[i] : 1.0;
     int piv[NN];
+    cplx M[NN][NN];  // M[k][i]: multiplier of row i in column k (i > k)
     HCG_UNROLL
     for (int j = 0; j < NN; j++) {
         cplx c[NN];
@@ -193,7 +195,7 @@ __device__ __noinline__ void lu_factor(Lane& L) {
         for (int k = 0; k < j; k++) {
             cplx ukj = c[k];
             HCG_UNROLL
-            for (int i = k + 1; i < NN; i++) c[i] = c_submul(F[IDX(i, k)], ukj, c[i]);
+            for (int i = k + 1; i < NN; i++) c[i] = c_submul(M[k][i], ukj, c[i]);
         }
         int kp = j;
         double amax = c_abs2(c[j]);
@@ -205,15 +207,22 @@ __device__ __noinline__ void lu_factor(Lane& L) {
         piv[j] = kp;
         L.ipiv[j] = kp;
         reg_swap(c, j, kp);
-        if (kp != j)
-            for (int k = 0; k < j; k++) { cplx t = F[IDX(j, k)]; F[IDX(j, k)] = F[IDX(kp, k)]; F[IDX(kp, k)] = t; }
+        HCG_UNROLL
+        for (int k = 0; k < j; k++) reg_swap(M[k], j, kp);
         if (amax != 0.0) {
             cplx d = c[j];
             HCG_UNROLL
             for (int i = j + 1; i < NN; i++) c[i] = c_div(c[i], d);
         }
         HCG_UNROLL
-        for (int i = 0; i < NN; i++) F[IDX(i, j)] = c[i];
+        for (int i = 0; i <= j; i++) F[IDX(i, j)] = c[i];  // U part: final
+        HCG_UNROLL
+        for (int i = j + 1; i < NN; i++) M[j][i] = c[i];
+    }
+    HCG_UNROLL
+    for (int k = 0; k < NN; k++) {
+        HCG_UNROLL
+        for (int i = k + 1; i < NN; i++) F[IDX(i, k)] = M[k][i];
     }
     L.factorized = 1;
 }
