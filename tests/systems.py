@@ -80,3 +80,91 @@ def tracker_param_system():
     """F = [x^2 - a, x*y - a + b] with parameters (a, b) (test/tracker_test.jl:5-27)."""
     (x, y), (a, b) = variables(2, 2)
     return System([x**2 - a, x * y - a + b], 2, 2)
+
+
+def _golden_vectors():
+    import json
+    from pathlib import Path
+    d = json.loads((Path(__file__).resolve().parent / "golden" / "param_cases.json").read_text())
+    return {case: {k: np.array([complex(a, b) for a, b in v]) for k, v in d[case].items()} for case in ("steiner", "four_bar")}
+
+
+def steiner():
+    """Conics tangent to five given conics (test/test_cases/steiner_higher_prec.jl:35-64):
+    unknowns a[1:5] (the tangent conic f = a1 x1^2 + a2 x1 x2 + a3 x2^2 + a4 x1 + a5 x2 + 1)
+    and the tangency points y[:, i]; parameters v[1:6, i], the coefficients of the i-th
+    given conic g_i.  Per conic: f(y_i) = 0, g_i(y_i) = 0, det[∇f(y_i) ∇g_i(y_i)] = 0.
+    Unknowns ordered [a; vec(y)], parameters vec(v) (column-major, as in the reference).
+    Returns (F, p, q, s_p, s_q) with the reference's vectors."""
+    xs, ps = variables(15, 30)
+    a = xs[:5]
+    y = [(xs[5 + 2 * i], xs[6 + 2 * i]) for i in range(5)]
+    eqs = []
+    for i in range(5):
+        c = ps[6 * i: 6 * i + 6]
+        y1, y2 = y[i]
+        f = a[0] * y1**2 + a[1] * y1 * y2 + a[2] * y2**2 + a[3] * y1 + a[4] * y2 + 1
+        g = c[0] * y1**2 + c[1] * y1 * y2 + c[2] * y2**2 + c[3] * y1 + c[4] * y2 + c[5]
+        f1, f2 = 2 * a[0] * y1 + a[1] * y2 + a[3], a[1] * y1 + 2 * a[2] * y2 + a[4]
+        g1, g2 = 2 * c[0] * y1 + c[1] * y2 + c[3], c[1] * y1 + 2 * c[2] * y2 + c[4]
+        eqs += [f, g, f1 * g2 - f2 * g1]
+    v = _golden_vectors()["steiner"]
+    return System(eqs, 15, 30), v["p"], v["q"], v["s_p"], v["s_q"]
+
+
+def four_bar():
+    """Four-bar linkage synthesis (test/test_cases/four_bar.jl:2-25): 24 unknowns
+    (x, a, y, b, x̂, â, ŷ, b̂, γ[1:8], γ̂[1:8]), 16 parameters (δ[1:8], δ̂[1:8]).
+    Returns (F, p, q, s) with the reference's vectors (s is given to 8 digits)."""
+    xs, ps = variables(24, 16)
+    x, a, y, b, xh, ah, yh, bh = xs[:8]
+    g, gh = xs[8:16], xs[16:24]
+    d, dh = ps[:8], ps[8:16]
+    D1 = [(ah * x - dh[i] * x) * g[i] + (a * xh - d[i] * xh) * gh[i] + (ah - xh) * d[i] + (a - x) * dh[i] - d[i] * dh[i]
+          for i in range(8)]
+    D2 = [(bh * y - dh[i] * y) * g[i] + (b * yh - d[i] * yh) * gh[i] + (bh - yh) * d[i] + (b - y) * dh[i] - d[i] * dh[i]
+          for i in range(8)]
+    D3 = [g[i] * gh[i] + g[i] + gh[i] for i in range(8)]
+    v = _golden_vectors()["four_bar"]
+    return System(D1 + D2 + D3, 24, 16), v["p"], v["q"], v["s"]
+
+
+def straight_line_case():
+    """test/tracker_test.jl:93-105: G = (0.2 + 0.4i) [x^2 - 1, y - 1], F = [x^2 + y^2 - 2.3,
+    2x + 3y - 4]; returns (G, F, starts)."""
+    (x, y), _ = variables(2)
+    F = System([x**2 + y**2 - 2.3, 2 * x + 3 * y - 4], 2)
+    G = System([(0.2 + 0.4j) * (x**2 - 1), (0.2 + 0.4j) * (y - 1)], 2)
+    return G, F, np.array([[1, 1], [-1, 1]], dtype=complex)
+
+
+def framework_454(seed: int = 454):
+    """test/tracker_test.jl:137-219 (issue 454): a bar framework with 11 vertices, 15 edges,
+    vertices 1, 6, 11 pinned, plus one linear slice a·x = b through the initial configuration.
+    The start configuration has a singular Jacobian.  Returns (F, b0, b1, start)."""
+    p0 = np.array([[0, 0], [1, 0], [1, 1], [2, 1], [1.5, 0.5], [3.0, 1.0], [4, 1], [5, 1], [4.5, 0.5], [5, 0],
+                   [6, 0]], dtype=float)
+    E = [(1, 2), (2, 3), (2, 5), (3, 4), (3, 5), (4, 5), (4, 6), (5, 9), (6, 7), (7, 8), (7, 9), (8, 9), (8, 10),
+         (9, 10), (10, 11)]
+    pinned = [1, 6, 11]
+    free = [2, 3, 4, 5, 7, 8, 9, 10]
+    xs, (b,) = variables(16, 1)
+    pos = {}
+    for j, i in enumerate(free):
+        pos[i] = (xs[2 * j], xs[2 * j + 1])
+    for i in pinned:
+        pos[i] = (float(p0[i - 1, 0]), float(p0[i - 1, 1]))
+    eqs = []
+    for i, j in E:
+        e = (pos[i][0] - pos[j][0]) ** 2 + (pos[i][1] - pos[j][1]) ** 2
+        e = e - float(((p0[i - 1] - p0[j - 1]) ** 2).sum())
+        eqs.append(e)
+    a = [0, 1.0] * 8
+    lin = sum(a[k] * xs[k] for k in range(16))
+    eqs.append(lin - b)
+    start = np.array([p0[i - 1, k] for i in free for k in range(2)], dtype=complex)
+    b0 = sum(a[k] * start[k] for k in range(16))
+    rng = np.random.default_rng(seed)
+    p1 = p0 + 1e-3 * rng.standard_normal(p0.shape)
+    b1 = sum(a[2 * j + k] * p1[i - 1, k] for j, i in enumerate(free) for k in range(2))
+    return System(eqs, 16, 1), np.array([b0]), np.array([b1]), start

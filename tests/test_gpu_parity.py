@@ -12,7 +12,7 @@ import oracle
 import systems
 from hcgpu import abi
 from hcgpu.configs import dense_system
-from hcgpu.homotopies import ParameterHomotopy, total_degree
+from hcgpu.homotopies import ParameterHomotopy, StraightLineHomotopy, total_degree
 from hcgpu.solve import EndgameTracker, make_options
 
 pytestmark = pytest.mark.gpu
@@ -118,3 +118,44 @@ def test_lane_schedule_does_not_change_results(gpu_ctx, refill, bpc):
     assert (g.accepted_steps == base.accepted_steps).all() and (g.rejected_steps == base.rejected_steps).all()
     sub = type(g)(g.records[:81], g.solution[:81], g.last_point[:81], g.valuation[:81])
     assert_parity(sub, oracle.track_batch(H, S[:81], make_options()))
+
+
+def _isapprox(a, b, rtol=np.sqrt(np.finfo(float).eps)):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_steiner_parameter_case(gpu_ctx):
+    """test/test_cases/steiner_higher_prec.jl on the GPU (n = 15: the generic-size linear
+    algebra): s_p at p reaches s_q at q, and s_q reaches s_p on the reversed homotopy, with
+    extended-precision residuals on the way; results equal the oracle's."""
+    F, p, q, s_p, s_q = systems.steiner()
+    for H, start, want in ((ParameterHomotopy(F, p, q), s_p, s_q), (ParameterHomotopy(F, q, p), s_q, s_p)):
+        g, o = run_both(H, start[None, :], gpu_ctx)
+        assert_parity(g, o)
+        assert g.codes() == ["success"]
+        assert _isapprox(g.solution[0], want)
+        assert g.extended_precision_used[0] == 1
+
+
+def test_four_bar_parameter_case(gpu_ctx):
+    """test/test_cases/four_bar.jl on the GPU (n = 24, the largest supported size): the
+    8-digit start point ends in success, as on the oracle."""
+    F, p, q, s = systems.four_bar()
+    g, o = run_both(ParameterHomotopy(F, p, q), s[None, :], gpu_ctx)
+    assert_parity(g, o)
+    assert g.codes() == ["success"]
+
+
+def test_straight_line_and_invalid_starts(gpu_ctx):
+    """test/tracker_test.jl:93-105 and :137-219 on the GPU: two successful starts, an invalid
+    start, and a start with a singular Jacobian, classified as the oracle classifies them."""
+    G, F, S = systems.straight_line_case()
+    S = np.vstack([S, [[100, -100]]]).astype(complex)
+    g, o = run_both(StraightLineHomotopy(G, F), S, gpu_ctx)
+    assert_parity(g, o)
+    assert g.codes() == ["success", "success", "terminated_invalid_startvalue"]
+    F, b0, b1, start = systems.framework_454()
+    g, o = run_both(ParameterHomotopy(F, b0, b1), start[None, :], gpu_ctx)
+    assert_parity(g, o)
+    assert g.codes() == ["terminated_invalid_startvalue_singular_jacobian"]

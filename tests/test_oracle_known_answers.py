@@ -15,7 +15,7 @@ import oracle
 import systems
 from hcgpu.configs import dense_system
 from hcgpu.expr import ExprSystem, expr_variables
-from hcgpu.homotopies import ParameterHomotopy, total_degree
+from hcgpu.homotopies import ParameterHomotopy, StraightLineHomotopy, total_degree
 from hcgpu.polys import System, variables
 from hcgpu.result import Result, path_results_from_batch
 from hcgpu.solve import make_options
@@ -65,6 +65,53 @@ def test_tracker_parameter_homotopy():
     back = oracle.track_plain(H, r["solution"], t1=0.0, t0=1.0)
     assert back["return_code"] == "success"
     np.testing.assert_allclose(back["solution"], [1, 1], rtol=1e-10)
+
+
+def _isapprox(a, b, rtol=np.sqrt(np.finfo(float).eps)):
+    """Julia's default isapprox on vectors: norm(a - b) <= rtol * max(norm(a), norm(b))."""
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_steiner_higher_precision():
+    """test/test_cases/steiner_higher_prec.jl: a parameter path close to a singularity
+    (extended-precision residuals for a step or two); s_p at p tracks to s_q at q and back."""
+    F, p, q, s_p, s_q = systems.steiner()
+    H = ParameterHomotopy(F, p, q)
+    r = oracle.track_plain(H, s_p, p=p, q=q)
+    assert r["return_code"] == "success"
+    assert _isapprox(r["solution"], s_q)
+    back = oracle.track_plain(H, r["solution"], t1=0.0, t0=1.0, p=p, q=q)
+    assert back["return_code"] == "success"
+    assert _isapprox(back["solution"], s_p)
+
+
+def test_four_bar():
+    """test/test_cases/four_bar.jl: iterative refinement in the predictor; the start point is
+    given to 8 digits only and the path still ends in success."""
+    F, p, q, s = systems.four_bar()
+    H = ParameterHomotopy(F, p, q)
+    r = oracle.track_plain(H, s, p=p, q=q)
+    assert r["return_code"] == "success"
+
+
+def test_tracker_straight_line_and_invalid_start():
+    """test/tracker_test.jl:93-105: both starts succeed; [100, -100] is an invalid start."""
+    G, F, S = systems.straight_line_case()
+    H = StraightLineHomotopy(G, F)
+    for s in S:
+        assert oracle.track_plain(H, s)["return_code"] == "success"
+    assert oracle.track_plain(H, [100, -100])["return_code"] == "terminated_invalid_startvalue"
+
+
+def test_tracker_invalid_start_singular_jacobian():
+    """test/tracker_test.jl:137-219 (issue 454): a framework configuration with a singular
+    Jacobian is classified as an invalid start with singular Jacobian."""
+    F, b0, b1, start = systems.framework_454()
+    assert max(abs(f(list(start) + list(b0))) for f in F.polys) == 0.0
+    H = ParameterHomotopy(F, b0, b1)
+    r = oracle.track_plain(H, start, p=b0, q=b1)
+    assert r["return_code"] == "terminated_invalid_startvalue_singular_jacobian"
 
 
 def test_cyclic7():
