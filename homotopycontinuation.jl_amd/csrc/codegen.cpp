@@ -16,6 +16,7 @@
 #include <array>
 #include <climits>
 #include <functional>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <sstream>
@@ -88,6 +89,36 @@ bool validate_tape(const hcg_tape* tp, int n, std::string* err) {
 enum ParamMode { PM_NONE, PM_FIXED, PM_DYNAMIC };
 
 // ---------------------------- ComplexF64 / DD eval ----------------------------
+// Instructions that recompute a value an earlier instruction of the same tape computed
+// (same operation on the same values, by value numbering).  A tape producer repeats cheap
+// products on purpose (rematerialization: shorter live ranges); the generated code passes
+// the operands of such an instruction through hcg_opaque so that the compiler does not
+// merge it back into the first computation, and its operations are not counted as
+// algorithmic flops.
+static std::vector<char> recomputed(const hcg_tape* tp) {
+    std::vector<long> vn(tp->tape_space + 1);
+    for (int sl = 0; sl <= tp->tape_space; sl++) vn[sl] = sl;
+    std::map<std::array<long, 5>, long> seen;
+    long next = tp->tape_space + 1;
+    std::vector<char> dup(tp->ninstr, 0);
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        const int op = I[4];
+        if (op == OP_STOP) break;
+        std::array<long, 5> key = {op, -1, -1, -1, -1};
+        for (int q = 0; q < arity(op); q++) key[q + 1] = (op == OP_POW_INT && q == 1) ? -2L - I[1] : vn[I[q]];
+        auto it = seen.find(key);
+        if (it != seen.end()) {
+            dup[i] = 1;
+            vn[I[5]] = it->second;
+        } else {
+            seen[key] = next;
+            vn[I[5]] = next++;
+        }
+    }
+    return dup;
+}
+
 // How an output is written: store(kind 'u' or 'U', index, value expression) -> statement.
 // The default writes the value; the fused straight-line Jacobian combines it with the
 // other system's output instead.
@@ -172,6 +203,7 @@ static long gen_eval_body(std::ostringstream& os, const hcg_tape* tp, bool jac, 
             os << "  " << store(st.kind, st.idx, nm[st.slot]) << "\n";
         }
     };
+    const std::vector<char> dup = recomputed(tp);
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
@@ -181,6 +213,16 @@ static long gen_eval_body(std::ostringstream& os, const hcg_tape* tp, bool jac, 
         std::string a = nm[I[0]], b = arity(op) > 1 && op != OP_POW_INT ? nm[I[1]] : "", c = arity(op) > 2 ? nm[I[2]] : "",
                     d = arity(op) > 3 ? nm[I[3]] : "";
         std::string out = pfx + "T" + std::to_string(i);
+        if (dup[i] && !dd) {
+            int k = 0;
+            for (std::string* opnd : {&a, &b, &c, &d}) {
+                if (opnd->empty() || opnd->compare(0, 4, "mkc(") == 0) continue;
+                std::string o = out + "o" + std::to_string(k++);
+                os << "  const cplx " << o << " = hcg_opaque(" << *opnd << ");\n";
+                *opnd = o;
+            }
+        }
+        const long f_before = f;
         std::string e;
         switch (op) {
         case OP_CB: e = std::string(pre) + "cb(" + a + ")"; f += dm * 2; break;
@@ -205,6 +247,7 @@ static long gen_eval_body(std::ostringstream& os, const hcg_tape* tp, bool jac, 
         case OP_MULMULADD: e = std::string(pre) + "mulmuladd(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 2 * dm + da; break;
         case OP_MULMULSUB: e = std::string(pre) + "mulmulsub(" + a + ", " + b + ", " + c + ", " + d + ")"; f += 2 * dm + da; break;
         }
+        if (dup[i]) f = f_before;
         os << "  const " << T << " " << out << " = " << e << ";\n";
         nm[I[5]] = out;
         flush_stores(i);
@@ -392,7 +435,9 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
     }
     os << "}\n";
     long f = pm == PM_DYNAMIC ? 6L * tp->nparams : 0;  // p(t) of every parameter
+    const std::vector<char> dup = recomputed(tp);  // code[i] is tape instruction i
     for (int i = 0; i < ninstr; i++) {
+        if (dup[i]) continue;
         int op = code[6 * i];
         switch (op) {
         case OP_MUL: f += 58; break;
@@ -506,6 +551,7 @@ static long gen_taylor_body(std::ostringstream& os, const hcg_tape* tp, int K, i
                << "\n";
         }
     };
+    const std::vector<char> tdup = recomputed(tp);
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
@@ -513,6 +559,29 @@ static long gen_taylor_body(std::ostringstream& os, const hcg_tape* tp, int K, i
         for (int q = 0; q < arity(op); q++)
             if (!(op == OP_POW_INT && q == 1)) need(I[q]);
         std::string base = pfx + "S" + std::to_string(i);
+        // a recomputation (see recomputed()): its operands' coefficients pass through
+        // hcg_opaque; the original series are put back before the result is recorded
+        std::vector<std::pair<int, SVal>> saved;
+        if (tdup[i]) {
+            int k = 0;
+            for (int q = 0; q < arity(op); q++) {
+                if (op == OP_POW_INT && q == 1) continue;
+                const int sl = I[q];
+                bool done = false;
+                for (auto& sv : saved) done |= sv.first == sl;
+                if (done) continue;
+                saved.push_back({sl, v[sl]});
+                SVal o = v[sl];
+                for (auto& cn : o.c) {
+                    if (cn.compare(0, 4, "mkc(") == 0) continue;
+                    std::string nmo = base + "o" + std::to_string(k++);
+                    os << "  const cplx " << nmo << " = hcg_opaque(" << cn << ");\n";
+                    cn = nmo;
+                }
+                v[sl] = o;
+            }
+        }
+        const long f_before = f;
         const SVal& a = v[I[0]];
         SVal r;
         auto emit = [&](int k, const std::string& e) {
@@ -623,6 +692,8 @@ static long gen_taylor_body(std::ostringstream& os, const hcg_tape* tp, int K, i
             break;
         }
         }
+        for (auto& sv : saved) v[sv.first] = sv.second;
+        if (tdup[i]) f = f_before;
         v[I[5]] = r;
         flush_stores(i);
     }

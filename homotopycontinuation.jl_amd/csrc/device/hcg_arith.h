@@ -398,6 +398,13 @@ HD double h_of(double a) { return 2 * a * (sqrt(4 * (a * a) + 1) - 2 * a); }
 // p - q of its Taylor series (tp!, src/homotopies/parameter_homotopy.jl:66-87).
 // Both forms are computed and one is selected (no branch): inlined for every parameter of
 // a straight-line body, a branch would split it into hundreds of blocks.
+// A copy of v that the compiler cannot identify with v: the generated code passes the
+// operands of a deliberately repeated computation through it, so that the repetition is not
+// merged back into the first computation (which would keep its result live in between).
+HD cplx hcg_opaque(cplx v) {
+    asm volatile("" : "+v"(v.re), "+v"(v.im));
+    return v;
+}
 HD cplx par_lerp(cplx t, const double* p, const double* q, int k) {
     cplx pk = mkc(p[2 * k], p[2 * k + 1]), qk = mkc(q[2 * k], q[2 * k + 1]);
     cplx rr = c_add(c_scale(t.re, pk), c_scale(1.0 - t.re, qk));

@@ -13,6 +13,7 @@ from __future__ import annotations
 from typing import Dict, List, Optional, Sequence, Tuple
 
 from .polys import Poly, System
+from .tape import _ARITY_OF as _ARITY
 from .tape import OP, Tape
 
 _INTERN: Dict[tuple, "Expr"] = {}
@@ -183,14 +184,23 @@ class ExprSystem(System):
         raise NotImplementedError("substitute parameters in the expressions instead")
 
 
+def _cheap(e: "Expr") -> bool:
+    """A product or power of leaves (variables, parameters, constants): one or two operations
+    to recompute, while keeping it live across the rows of a system costs registers."""
+    return e.op in ("mul", "pow") and all(a.op in ("var", "par", "const") for a in e.args)
+
+
 class _Lower:
-    def __init__(self, nvars, nparams):
+    def __init__(self, nvars, nparams, remat=False):
         self.nvars, self.nparams = nvars, nparams
         self.consts: List[complex] = []
         self.cidx: Dict[complex, int] = {}
         self.instrs: List[Tuple[str, tuple]] = []
         self.ref: Dict[tuple, tuple] = {}
         self.uses: Dict[tuple, int] = {}
+        # remat == "use": cheap nodes are not shared but recomputed, bit for bit the same
+        # operations, at every use
+        self.remat = remat
 
     def const(self, c):
         c = complex(c)
@@ -213,7 +223,7 @@ class _Lower:
         return ("i", len(self.instrs) - 1)
 
     def lower(self, e: Expr):
-        if e._key in self.ref:
+        if e._key in self.ref and not (self.remat == "use" and _cheap(e)):
             return self.ref[e._key]
         op = e.op
         if op == "const":
@@ -224,11 +234,15 @@ class _Lower:
             r = ("p", e.val)
         elif op == "add":
             a, b = e.args
-            # fuse a*b + c when the product is used only here
+            # fuse a*b + c when the product is used only here; the addend (typically the
+            # running sum of a long chain) is lowered first, so that the factors of each term
+            # are formed right before their multiply-add instead of all ahead of the chain
             if b.op == "mul" and self.uses.get(b._key, 0) == 1 and b._key not in self.ref:
-                r = self.emit("MULADD", self.lower(b.args[0]), self.lower(b.args[1]), self.lower(a))
+                acc = self.lower(a)
+                r = self.emit("MULADD", self.lower(b.args[0]), self.lower(b.args[1]), acc)
             elif a.op == "mul" and self.uses.get(a._key, 0) == 1 and a._key not in self.ref:
-                r = self.emit("MULADD", self.lower(a.args[0]), self.lower(a.args[1]), self.lower(b))
+                acc = self.lower(b)
+                r = self.emit("MULADD", self.lower(a.args[0]), self.lower(a.args[1]), acc)
             else:
                 r = self.emit("ADD", self.lower(a), self.lower(b))
         elif op == "sub":
@@ -277,26 +291,69 @@ class _Lower:
         return Tape(rows, self.consts, self.nparams, self.nvars, u, U, output_dim)
 
 
+def _live_peak(t: Tape) -> int:
+    """Peak number of live complex values of the code generated from `t`: inputs live from
+    their first to their last use, temporaries from their definition to their last use, and
+    an output slot until the instruction that writes it last (it is stored right there)."""
+    rows = [r for r in t.instr if r[4] != OP["STOP"]]
+    first_input = t.nconst + 1
+    first_use: Dict[int, int] = {}
+    last_use: Dict[int, int] = {}
+    start: Dict[int, int] = {}
+    for i, r in enumerate(rows):
+        for q in range(_ARITY[int(r[4])]):
+            if r[4] == OP["POW_INT"] and q == 1:
+                continue
+            if r[q] >= first_input:
+                last_use[r[q]] = i
+                first_use.setdefault(r[q], i)
+        start.setdefault(r[5], i)
+        last_use[r[5]] = max(last_use.get(r[5], i), i)
+    delta = [0] * (len(rows) + 2)
+    for sl, lu in last_use.items():
+        delta[start.get(sl, first_use.get(sl, lu))] += 1
+        delta[lu + 1] -= 1
+    live = peak = 0
+    for d in delta:
+        live += d
+        peak = max(peak, live)
+    return peak
+
+
 def expr_tapes(system: ExprSystem) -> Tuple[Tape, Tape]:
+    """(evaluate_and_jacobian! tape, evaluate! tape).  Emission order: every row at once
+    with all common subexpressions shared, or row by row with products of leaves recomputed
+    at every use (identical operations, so identical values; the code generator keeps the
+    recomputations apart), whichever keeps fewer values live."""
     n, np_ = system.nvars, system.nparams
     memo: dict = {}
     J = [[differentiate(e, j, memo) for j in range(n)] for e in system.exprs]
-    # evaluate_and_jacobian! tape
-    Lj = _Lower(n, np_)
-    seen: set = set()
-    for e in system.exprs:
-        Lj.count_uses(e, seen)
-    for row in J:
-        for d in row:
-            Lj.count_uses(d, seen)
-    outs = [None if e.is_const(0) else Lj.lower(e) for e in system.exprs]
-    jac = [[None if d.is_const(0) else Lj.lower(d) for d in row] for row in J]
-    ej = Lj.finish(outs, jac, len(system.exprs))
-    # evaluate! tape
-    Le = _Lower(n, np_)
-    seen = set()
-    for e in system.exprs:
-        Le.count_uses(e, seen)
-    outs = [None if e.is_const(0) else Le.lower(e) for e in system.exprs]
-    ev = Le.finish(outs, None, len(system.exprs))
-    return ej, ev
+    m = len(system.exprs)
+
+    def lowering(exprs_for_uses, remat):
+        L = _Lower(n, np_, remat)
+        seen: set = set()
+        for e in exprs_for_uses:
+            L.count_uses(e, seen)
+        return L
+
+    def ej(remat):
+        L = lowering(list(system.exprs) + [d for row in J for d in row], remat)
+        if not remat:
+            outs = [None if e.is_const(0) else L.lower(e) for e in system.exprs]
+            jac = [[None if d.is_const(0) else L.lower(d) for d in row] for row in J]
+            return L.finish(outs, jac, m)
+        outs, jac = [], []
+        for e, row in zip(system.exprs, J):
+            outs.append(None if e.is_const(0) else L.lower(e))
+            jac.append([None if d.is_const(0) else L.lower(d) for d in row])
+        return L.finish(outs, jac, m)
+
+    def ev(remat):
+        L = lowering(list(system.exprs), remat)
+        return L.finish([None if e.is_const(0) else L.lower(e) for e in system.exprs], None, m)
+
+    def best(a, b):  # recomputation charged at one live value per 100 extra instructions
+        return min((a, b), key=lambda t: _live_peak(t) + (len(t.instr) - min(len(a.instr), len(b.instr))) / 100.0)
+
+    return best(ej(False), ej("use")), best(ev(False), ev("use"))

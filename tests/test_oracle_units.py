@@ -460,3 +460,58 @@ def test_tape_emission_orders_are_equivalent():
         t0 = oracle.sys_taylor(variants[4], 3, tx, p if S.nparams else None, dp if S.nparams else None)
         t1 = oracle.sys_taylor(variants[5], 3, tx, p if S.nparams else None, dp if S.nparams else None)
         assert np.array_equal(t0, t1)
+
+
+def test_expression_tape_rematerialization_is_equivalent():
+    """Expression systems: the tape with shared subexpressions and the tape that recomputes
+    products of leaves at every use evaluate, differentiate and expand (Taylor K = 1..3)
+    bit for bit identically; the code generator counts the recomputations' flops once."""
+    import ctypes
+    from hcgpu import abi, expr
+    from hcgpu.configs import shared_factor_exprs
+    from hcgpu.homotopies import total_degree
+    E = shared_factor_exprs(6, seed=3)
+    n = E.nvars
+
+    def lowered(remat):
+        memo = {}
+        J = [[expr.differentiate(e, j, memo) for j in range(n)] for e in E.exprs]
+        Lj = expr._Lower(n, 0, remat)
+        seen = set()
+        for e in list(E.exprs) + [d for r in J for d in r]:
+            Lj.count_uses(e, seen)
+        ej = Lj.finish([Lj.lower(e) for e in E.exprs], [[None if d.is_const(0) else Lj.lower(d) for d in r] for r in J],
+                       n)
+        Le = expr._Lower(n, 0, remat)
+        seen = set()
+        for e in E.exprs:
+            Le.count_uses(e, seen)
+        return ej, Le.finish([Le.lower(e) for e in E.exprs], None, n)
+
+    (ej0, ev0), (ej1, ev1) = lowered(False), lowered("use")
+    assert len(ej1.instr) > len(ej0.instr) and len(ev1.instr) > len(ev0.instr)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    u0, U0 = oracle.sys_eval(ej0, x)
+    u1, U1 = oracle.sys_eval(ej1, x)
+    assert np.array_equal(u0, u1) and np.array_equal(U0, U1)
+    assert np.array_equal(oracle.sys_eval(ev0, x)[0], oracle.sys_eval(ev1, x)[0])
+    for K in (1, 2, 3):
+        tx = rng.standard_normal((K, n)) + 1j * rng.standard_normal((K, n))
+        assert np.array_equal(oracle.sys_taylor(ev0, K, tx), oracle.sys_taylor(ev1, K, tx))
+    # the chosen tapes: counted flops equal those of the shared tape
+    assert expr._live_peak(expr.expr_tapes(E)[1]) <= expr._live_peak(ev0)
+    H, _ = total_degree(E, seed=1)
+    L = abi.lib()
+    need, fl = ctypes.c_size_t(), (ctypes.c_long * 6)()
+    abi.check(L.hcg_generate(ctypes.byref(H.desc()), None, 0, ctypes.byref(need), fl))
+    orig = expr.expr_tapes
+    try:
+        expr.expr_tapes = lambda system: lowered(False)
+        import hcgpu.homotopies as hm
+        H0, _ = hm.total_degree(shared_factor_exprs(6, seed=3), seed=1)
+        fl0 = (ctypes.c_long * 6)()
+        abi.check(L.hcg_generate(ctypes.byref(H0.desc()), None, 0, ctypes.byref(need), fl0))
+    finally:
+        expr.expr_tapes = orig
+    assert list(fl) == list(fl0)
