@@ -204,9 +204,12 @@ def bench_param10(args, out):
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
         import oracle  # CPU baseline only (never the measured GPU path)
+        from hcgpu.tape import shared_lowering
+        with shared_lowering():  # the CPU evaluates tapes with every subexpression shared
+            Hc = ParameterHomotopy(F, p0, Q, lift_affine=True)
         m = args.cpu_targets * len(S)
         t0 = time.perf_counter()
-        o = oracle.track_batch(H, starts[:m], plain_o, nthreads=args.cpu_threads, group_of_path=groups[:m])
+        o = oracle.track_batch(Hc, starts[:m], plain_o, nthreads=args.cpu_threads, group_of_path=groups[:m])
         dt = time.perf_counter() - t0
         okc = (o.code == 1) & (~o.singular)
         relc = np.abs(sol_l[:m][okc] - o.solution[okc]).max(axis=1) / np.maximum(1.0, np.abs(o.solution[okc]).max(axis=1))
@@ -241,8 +244,11 @@ def bench_dense8(args, out):
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
         import oracle  # CPU baseline only
+        from hcgpu.tape import shared_lowering
+        with shared_lowering():  # the CPU evaluates tapes with every subexpression shared
+            Hc, _ = total_degree(dense_system(8, 3, seed=1), seed=2)
         t0 = time.perf_counter()
-        o = oracle.track_batch(H, S, make_options(), nthreads=args.cpu_threads)
+        o = oracle.track_batch(Hc, S, make_options(), nthreads=args.cpu_threads)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(npaths / dt, 1), "unit": "paths/s", "cores": args.cpu_threads,
                                "kind": "port", "sample": f"all {npaths} paths, oracle, {dt:.1f} s"}
@@ -288,8 +294,13 @@ def bench_mixed(args, out):
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
         import oracle  # CPU baseline only
+        # the oracle evaluates the same systems with their common subexpressions shared (the
+        # GPU tapes recompute some products to save registers; both give identical values)
+        from hcgpu.tape import shared_lowering
+        with shared_lowering():
+            cpu_probs = cfg4_problem()
         cpu_s, sample = 0.0, []
-        for (H, S), g in zip(probs, louts):
+        for (H, S), g in zip(cpu_probs, louts):
             m = min(len(S), 2048)
             t0 = time.perf_counter()
             o = oracle.track_batch(H, S[:m], make_options(), nthreads=args.cpu_threads)

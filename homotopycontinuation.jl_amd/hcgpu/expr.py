@@ -356,4 +356,7 @@ def expr_tapes(system: ExprSystem) -> Tuple[Tape, Tape]:
     def best(a, b):  # recomputation charged at one live value per 100 extra instructions
         return min((a, b), key=lambda t: _live_peak(t) + (len(t.instr) - min(len(a.instr), len(b.instr))) / 100.0)
 
+    from .tape import _MODE
+    if _MODE[-1] == "cpu":
+        return ej(False), ev(False)
     return best(ej(False), ej("use")), best(ev(False), ev("use"))

@@ -319,9 +319,27 @@ def max_live(t: Tape) -> int:
     return peak
 
 
+# "gpu": emission order by register pressure (below); "cpu": the fewest instructions, i.e.
+# every common subexpression shared (what a CPU evaluator wants; set by `shared_lowering`)
+_MODE = ["gpu"]
+
+
+class shared_lowering:
+    """Context manager: systems lowered inside share every common subexpression (the fewest
+    instructions) instead of recomputing some to save registers.  Values are identical."""
+
+    def __enter__(self):
+        _MODE.append("cpu")
+
+    def __exit__(self, *exc):
+        _MODE.pop()
+
+
 def _best(variants):
     """The tape with the smallest register pressure among equivalent emission orders (peak
     live values, with recomputation charged at one live value per 50 extra instructions)."""
+    if _MODE[-1] == "cpu":
+        return min((make() for make in variants), key=lambda t: len(t.instr))
     tapes_ = [make() for make in variants]
     nmin = min(len(t.instr) for t in tapes_)
     scored = [(max_live(t) + (len(t.instr) - nmin) / 50.0, k, t) for k, t in enumerate(tapes_)]
