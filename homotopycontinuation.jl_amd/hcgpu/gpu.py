@@ -139,8 +139,10 @@ assert _REC_DTYPE.itemsize == ctypes.sizeof(hcg_path_result)
 
 
 def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, target_groups=None,
-                group_of_path=None, log_capacity: int = 0) -> BatchOutput:
-    """Track all `starts` (npaths x n complex) from t=1 to t=0 on the GPU."""
+                group_of_path=None, log_capacity: int = 0, start_params=None) -> BatchOutput:
+    """Track all `starts` (npaths x n complex) from t=1 to t=0 on the GPU.  For a parameter
+    homotopy, `start_params` / `target_groups` replace the homotopy's own p / q (in the
+    space its tapes use, see ParameterHomotopy.target_matrix)."""
     opts = opts or abi.default_options()
     starts = np.ascontiguousarray(np.asarray(starts, dtype=np.complex128))
     npaths, n = starts.shape
@@ -153,7 +155,9 @@ def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, ta
     keep = [starts]
     H = program.homotopy
     if getattr(H, "q", None) is not None:
-        p = np.ascontiguousarray(H.p)
+        p = np.ascontiguousarray(H.p if start_params is None else np.asarray(start_params, dtype=np.complex128))
+        if p.shape != np.shape(H.p):
+            raise ValueError("start_params has the wrong length")
         q = np.ascontiguousarray(H.q if target_groups is None else np.asarray(target_groups, dtype=np.complex128))
         keep += [p, q]
         b.start_params, b.target_params = _dp(p), _dp(q)

@@ -1,0 +1,56 @@
+"""Monodromy on the GPU: every loop edge is one libhcgpu launch over all known solutions.
+
+* the same monodromy run (same seed) on the GPU and on the oracle's tracker gives the same
+  solutions in the same order (the trackers agree bit for bit);
+* from ONE of the parameter workload's start solutions, monodromy at p0 recovers all 1,024
+  committed start solutions (tests/golden/cfg2_starts.npy, produced by a total-degree solve).
+"""
+import time
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from hcgpu.configs import cfg2_problem, sparse_param_system
+from hcgpu.monodromy import _new_points, gpu_edge_tracker, monodromy_solve
+from hcgpu.solve import make_options
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+class _Edge:
+    def __init__(self, H, p, q):
+        self._H, self.p, self.q = H, p, q[None, :]
+
+    def desc(self):
+        return self._H.desc()
+
+
+def _oracle_tracker(H, X, p, q):
+    o = oracle.track_batch(_Edge(H, p, q), X, make_options())
+    return o.code, o.solution
+
+
+def test_monodromy_gpu_equals_oracle(gpu_ctx):
+    F = sparse_param_system(3, 6, 4, seed=7)
+    g = monodromy_solve(F, seed=11, tracker=gpu_edge_tracker(gpu_ctx), target_solutions_count=8)
+    o = monodromy_solve(F, seed=11, tracker=_oracle_tracker, target_solutions_count=8)
+    assert g.nsolutions() == o.nsolutions() == 8
+    assert g.new_per_loop == o.new_per_loop
+    np.testing.assert_array_equal(g.solutions, o.solutions)
+
+
+def test_monodromy_recovers_parameter_workload_starts(gpu_ctx):
+    F, p0, _ = cfg2_problem(1)
+    S = np.load(GOLDEN / "cfg2_starts.npy")
+    t0 = time.perf_counter()
+    r = monodromy_solve(F, S[:1], p0, target_solutions_count=len(S), lift_affine=True, seed=0,
+                        tracker=gpu_edge_tracker(gpu_ctx), max_loops=80)
+    dt = time.perf_counter() - t0
+    print(f"monodromy: {r.nsolutions()} solutions in {r.loops} loops, {r.tracked_paths} paths, {dt:.2f} s; "
+          f"new per loop {r.new_per_loop}")
+    assert r.returncode == "success" and r.nsolutions() == len(S)
+    assert len(_new_points(S, r.solutions, 1e-8)) == 0 and len(_new_points(r.solutions, S, 1e-8)) == 0

@@ -1,0 +1,77 @@
+"""Monodromy start-solution generation (src/monodromy.jl), driven by the oracle's tracker on
+the CPU and by libhcgpu on the GPU.
+
+Pinned by: the closed-form solutions of test/tracker_test.jl's system (x = ±√a), and the
+total-degree solve of the same system at the same parameters (all nonsingular endpoints,
+complete by Bézout for generic coefficients): monodromy must find exactly that set.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import systems
+from hcgpu.configs import sparse_param_system
+from hcgpu.homotopies import total_degree
+from hcgpu.monodromy import _new_points, find_start_pair, monodromy_solve
+from hcgpu.polys import System, variables
+from hcgpu.solve import make_options
+
+
+class _Edge:
+    """The template homotopy with the edge's start and end parameters."""
+
+    def __init__(self, H, p, q):
+        self._H, self.p, self.q = H, p, q[None, :]
+
+    def desc(self):
+        return self._H.desc()
+
+
+def oracle_tracker(H, X, p, q):
+    o = oracle.track_batch(_Edge(H, p, q), X, make_options(), nthreads=4)
+    return o.code, o.solution
+
+
+def same_set(A, B, tol=1e-8):
+    if len(A) != len(B):
+        return False
+    return len(_new_points(A, B, tol)) == 0 and len(_new_points(B, A, tol)) == 0
+
+
+def test_find_start_pair_is_a_solution():
+    F = sparse_param_system(3, 6, 4, seed=7)
+    x0, p0 = find_start_pair(F, np.random.default_rng(1))
+    res = max(abs(f(list(x0) + list(p0))) for f in F.polys)
+    assert res < 1e-12
+    with pytest.raises(ValueError):  # Steiner: the tangent-conic equations have no parameters
+        find_start_pair(systems.steiner()[0], np.random.default_rng(2))
+    (x,), (a,) = variables(1, 1)
+    with pytest.raises(ValueError):
+        find_start_pair(System([x**2 - a * a], 1, 1), np.random.default_rng(1))
+
+
+def test_monodromy_closed_form():
+    """F = [x^2 - a, xy - a + b]: from one start pair both solutions x = ±√a, y = (a - b)/x."""
+    F = systems.tracker_param_system()
+    r = monodromy_solve(F, seed=3, tracker=oracle_tracker)
+    a, b = r.parameters
+    want = np.array([[s * np.sqrt(a), (a - b) / (s * np.sqrt(a))] for s in (1, -1)])
+    assert r.nsolutions() == 2
+    assert same_set(r.solutions, want, 1e-10)
+    assert r.returncode == "heuristic_stop" and r.new_per_loop[-5:] == [0] * 5
+
+
+def test_monodromy_equals_total_degree_solve():
+    """3 quadrics in 3 unknowns, coefficients affine in 4 parameters: monodromy from one start
+    pair finds the 8 solutions the total-degree solve finds at the same parameters."""
+    F = sparse_param_system(3, 6, 4, seed=7)
+    r = monodromy_solve(F, seed=11, tracker=oracle_tracker, target_solutions_count=8)
+    assert r.returncode == "success" and r.nsolutions() == 8
+    Fp = F.fix_parameters(r.parameters)
+    H, S = total_degree(Fp, seed=5)
+    o = oracle.track_batch(H, S, make_options())
+    ok = (o.code == 1) & (~o.singular)
+    assert ok.sum() == 8
+    assert same_set(r.solutions, o.solution[ok], 1e-8)
+    for x in r.solutions:
+        assert max(abs(f(list(x) + list(r.parameters))) for f in F.polys) < 1e-10
