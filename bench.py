@@ -151,6 +151,46 @@ def learned_weights(ctx, F, p0, S):
     return w, f"trained now on {info['records']} step records (loss {loss:.4f})"
 
 
+def certify_leg(ctx, F, Q, sol, rec, groups, args):
+    """Interval Krawczyk certification of every successful endpoint of the learned run, each
+    against its own target parameters, in one grouped launch (outside the timed region)."""
+    import numpy as np
+    from hcgpu.certify import certify_groups
+    ok = rec["code"] == 1
+    S, G = sol[ok], groups[ok]
+    cert = ctx.certifier(F)
+    t0 = time.perf_counter()
+    r = certify_groups(F, S, Q, G, certifier=cert)
+    wall = time.perf_counter() - t0
+    kms = ctx.last_kernel_ms()
+    prec = np.array([c.prec for c in r.certificates])
+    leg = {"candidates": int(len(S)), "certified": r.ncertified(), "distinct_certified": r.ndistinct_certified(),
+           "by_double_double_retry": int(((prec == 106) & np.array([c.certified for c in r.certificates])).sum()),
+           "kernel": "hcg_certify_kernel", "kernel_ms": round(kms, 3),
+           "value": round(len(S) / (kms * 1e-3), 1), "unit": "certificates/s",
+           "host_call_s": round(wall, 3)}
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        import oracle  # CPU baseline only (never the measured GPU path)
+        from hcgpu.certify import parameter_boxes
+        from hcgpu.homotopies import _SysTapes
+        m = min(len(S), 4096)
+        enc = [parameter_boxes(list(q)) for q in Q]
+        pts = np.stack([e[0] for e in enc])
+        tapes = _SysTapes(F)
+        t0 = time.perf_counter()
+        o = oracle.certify(tapes.struct(), F.nvars, S[:m], pts, None, None, True, True, False, args.cpu_threads,
+                           group_of=G[:m])
+        dt = time.perf_counter() - t0
+        g = cert.certify(S[:m], pts, real_system=False, group_of=G[:m])
+        same = all(np.array_equal(getattr(g, f), getattr(o, f)) for f in ("certified", "real", "complex_", "prec",
+                                                                           "contraction"))
+        leg["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "certificates/s", "cores": args.cpu_threads,
+                               "kind": "port", "sample": f"first {m} candidates, oracle, {dt:.2f} s"}
+        leg["parity_vs_oracle_sample"] = {"identical_records_and_boxes": bool(same and np.array_equal(g.box1, o.box1))}
+    return leg
+
+
 def bench_param10(args, out):
     import numpy as np
     from hcgpu import learned
@@ -201,6 +241,8 @@ def bench_param10(args, out):
         "parity_learned_vs_plain": {"identical_return_codes": bool((rec_l["code"] == rec_p["code"]).all()),
                                     "max_rel_endpoint_diff": float(rel.max())},
     })
+    if not args.no_certify:
+        out["certification"] = certify_leg(ctx, F, Q, sol_l, rec_l, groups, args)
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
         import oracle  # CPU baseline only (never the measured GPU path)
@@ -322,6 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-targets", type=int, default=32, help="targets of the CPU baseline sample (param10)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -309,26 +309,32 @@ static long gen_straight_eval_jac(std::ostringstream& os, const hcg_system& G, c
 // instead of being unrolled: the tape is re-encoded with its slots compacted by a
 // linear scan (operand >= 0: register slot, < 0: constant -k-1) and stored in
 // __constant__ tables; each lane keeps its live slots in a private array.
-static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, int n, ParamMode pm,
-                          const double* fixed, long* flops) {
-    const int base = tp->nconst + tp->nparams + tp->nvars;
-    std::vector<double> consts;
-    for (int k = 0; k < 2 * tp->nconst; k++) consts.push_back(tp->constants[k]);
-    // operand of every tape slot at the current point: >=0 register, <0 constant
-    std::vector<int> where(tp->tape_space + 1, 0);
-    std::vector<int> last_use(tp->tape_space + 1, -1);
+// An interpreter program for one tape: 6 ints per instruction (op, four operands, result
+// register), operands >= 0 registers, < 0 constants -e-1, >= PBASE parameter e - PBASE;
+// registers assigned by a linear scan over the tape's values (the instructions are SSA after
+// validation except for slot reuse, so liveness is tracked per definition).
+struct InterpPlan {
+    std::vector<int> code;
+    std::vector<double> consts;         // (re, im) pairs: the tape's constants, then fixed parameters
+    std::vector<std::pair<int, int>> loads;  // (register, input index) of the variables read
+    std::vector<int> out_operand;       // per output: its operand, INT32_MIN for a zero output
+    int nreg = 0, ninstr = 0;
+};
+
+static InterpPlan plan_interp(const hcg_tape* tp, const std::vector<std::pair<int, int>>& outputs, int nout,
+                              ParamMode pm, const double* fixed, int PBASE) {
+    InterpPlan P;
+    for (int k = 0; k < 2 * tp->nconst; k++) P.consts.push_back(tp->constants[k]);
     int ninstr = 0;
     for (int i = 0; i < tp->ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         if (I[4] == OP_STOP) break;
         ninstr++;
     }
-    // last use of each (slot, definition) approximated per slot: the instructions are SSA
-    // after validation except for slot reuse, so track uses per definition index
+    P.ninstr = ninstr;
     std::vector<int> def_of(tp->tape_space + 1, -1);   // current definition id of slot
     std::vector<int> def_last(ninstr + tp->nparams + tp->nvars + 1, -1);
     int ndefs = 0;
-    std::vector<int> input_def(tp->tape_space + 1, -1);
     for (int k = 0; k < tp->nparams + tp->nvars; k++) {
         int slot = tp->params_first + k;
         def_of[slot] = ndefs++;
@@ -351,38 +357,33 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
     }
     def_last.resize(ndefs, -1);
     // outputs keep their definitions alive to the end
-    std::vector<int> out_operand(n, INT32_MIN);
-    std::vector<int> out_is_def(n, -1), out_const(n, -1);
-    for (int i = 0; i < tp->nu; i++) {
-        int row = tp->u_assign[2 * i] - 1, s = tp->u_assign[2 * i + 1];
-        if (s <= tp->nconst) out_const[row] = s - 1;
-        else { out_is_def[row] = def_of[s]; def_last[def_of[s]] = ninstr; }
+    std::vector<int> out_is_def(nout, -1), out_const(nout, -1);
+    for (auto& o : outputs) {
+        int idx = o.first, s = o.second;
+        if (s <= tp->nconst) out_const[idx] = s - 1;
+        else { out_is_def[idx] = def_of[s]; def_last[def_of[s]] = ninstr; }
     }
-    // linear scan register assignment
     std::vector<int> reg(ndefs, -1);
     std::vector<int> free_regs;
     int nreg = 0;
     auto alloc = [&]() { if (!free_regs.empty()) { int r = free_regs.back(); free_regs.pop_back(); return r; } return nreg++; };
-    // parameters are not copied into registers: fixed ones become constants, homotopy-driven
-    // ones are fetched as t p + (1 - t) q when an instruction reads them (operand DD_PBASE + k)
-    const int DD_PBASE = 1 << 24;
+    // parameters are not copied into registers: fixed ones become constants, the others are
+    // fetched when an instruction reads them (operand PBASE + k)
     std::vector<int> par_operand(tp->nparams, 0);
     for (int k = 0; k < tp->nparams; k++) {
         if (pm == PM_FIXED) {
-            consts.push_back(fixed[2 * k]);
-            consts.push_back(fixed[2 * k + 1]);
-            par_operand[k] = -(int)(consts.size() / 2) ;
+            P.consts.push_back(fixed[2 * k]);
+            P.consts.push_back(fixed[2 * k + 1]);
+            par_operand[k] = -(int)(P.consts.size() / 2) ;
         } else {
-            par_operand[k] = DD_PBASE + k;
+            par_operand[k] = PBASE + k;
         }
     }
-    std::vector<std::pair<int, int>> loads;  // (register, input index) for vars
     for (int k = tp->nparams; k < tp->nparams + tp->nvars; k++) {
         if (def_last[k] < 0) continue;  // unused input
         reg[k] = alloc();
-        loads.push_back({reg[k], k});
+        P.loads.push_back({reg[k], k});
     }
-    std::vector<int> code;
     for (int i = 0; i < ninstr; i++) {
         const int32_t* I = tp->instr + 6 * i;
         int op = I[4];
@@ -404,23 +405,40 @@ static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std:
             }
         }
         int od = out_def[i];
-        int outr = def_last[od] >= 0 ? alloc() : alloc();
+        int outr = alloc();
         reg[od] = outr;
         if (def_last[od] < 0) free_regs.push_back(outr);  // dead value: slot reusable right away
-        code.insert(code.end(), {op, ops[0], ops[1], ops[2], ops[3], outr});
+        P.code.insert(P.code.end(), {op, ops[0], ops[1], ops[2], ops[3], outr});
     }
-    for (int r = 0; r < n; r++) {
-        if (out_is_def[r] >= 0) out_operand[r] = out_is_def[r] < tp->nparams ? par_operand[out_is_def[r]] : reg[out_is_def[r]];
-        else if (out_const[r] >= 0) out_operand[r] = -out_const[r] - 1;
+    P.out_operand.assign(nout, INT32_MIN);
+    for (int r = 0; r < nout; r++) {
+        if (out_is_def[r] >= 0) P.out_operand[r] = out_is_def[r] < tp->nparams ? par_operand[out_is_def[r]] : reg[out_is_def[r]];
+        else if (out_const[r] >= 0) P.out_operand[r] = -out_const[r] - 1;
     }
-    // fixed parameters become constants: rewrite their register loads
+    P.nreg = nreg;
+    return P;
+}
+
+static void emit_tables(std::ostringstream& os, const std::string& fname, const InterpPlan& P) {
     os << "__constant__ int " << fname << "_code[] = {";
-    for (size_t k = 0; k < code.size(); k++) os << (k ? "," : "") << code[k];
-    if (code.empty()) os << "0";
+    for (size_t k = 0; k < P.code.size(); k++) os << (k ? "," : "") << P.code[k];
+    if (P.code.empty()) os << "0";
     os << "};\n__constant__ double " << fname << "_const[] = {";
-    for (size_t k = 0; k < consts.size(); k++) os << (k ? "," : "") << dlit(consts[k]);
-    if (consts.empty()) os << "0.0";
+    for (size_t k = 0; k < P.consts.size(); k++) os << (k ? "," : "") << dlit(P.consts[k]);
+    if (P.consts.empty()) os << "0.0";
     os << "};\n";
+}
+
+static void gen_dd_interp(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, int n, ParamMode pm,
+                          const double* fixed, long* flops) {
+    std::vector<std::pair<int, int>> outs;
+    for (int i = 0; i < tp->nu; i++) outs.push_back({tp->u_assign[2 * i] - 1, tp->u_assign[2 * i + 1]});
+    const InterpPlan P = plan_interp(tp, outs, n, pm, fixed, 1 << 24);
+    const std::vector<int>& code = P.code;
+    const int ninstr = P.ninstr, nreg = P.nreg;
+    const auto& loads = P.loads;
+    const auto& out_operand = P.out_operand;
+    emit_tables(os, fname, P);
     os << "__device__ __noinline__ void " << fname << "(const cdd* __restrict__ x, ";
     if (pm == PM_DYNAMIC) os << "const double* __restrict__ pp, const double* __restrict__ pq, cplx pt, ";
     os << "cdd* __restrict__ u) {\n";
@@ -881,6 +899,66 @@ bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::st
     atomicAdd(&m.sys[0].counters[3], (unsigned long long)T.lacc);
 }
 )";
+    *src = os.str();
+    return true;
+}
+
+
+// ---------------------------- certification ----------------------------
+// Interval (ball = false) or double-double ball (ball = true) evaluation of a System tape:
+// the u outputs (jac = false) or the U outputs (jac = true), at a box X (IComplex) or at a
+// double-double point X, with parameter boxes / balls PB.  An interpreter program (the same
+// register planning as the double-double residual) rather than straight-line code: interval
+// and ball operations are long, and straight-line bodies of large tapes took tens of minutes
+// to compile.  ci_interpret / cb_interpret (hcg_interval.h) run the generic IComplex
+// operations; quotients have no ball form (their results are unbounded balls).
+static void gen_cert_interp(std::ostringstream& os, const hcg_tape* tp, const std::string& fname, bool jac, int n,
+                            ParamMode pm, const double* fixed, bool ball) {
+    const int nout = jac ? n * n : n, na = jac ? tp->nU : tp->nu;
+    const int32_t* as = jac ? tp->U_assign : tp->u_assign;
+    std::vector<std::pair<int, int>> outs;
+    for (int i = 0; i < na; i++) outs.push_back({as[2 * i] - 1, as[2 * i + 1]});
+    const InterpPlan P = plan_interp(tp, outs, nout, pm, fixed, 1 << 24);
+    emit_tables(os, fname, P);
+    const std::string T = ball ? "cball" : "icplx", pre = ball ? "cb" : "ci";
+    os << "__device__ __noinline__ void " << fname << "(const " << (ball ? "cdd" : "icplx")
+       << "* __restrict__ X, const double* __restrict__ PB, " << T << "* __restrict__ out) {\n";
+    os << "  " << T << " R[" << std::max(P.nreg, 1) << "];\n";
+    for (auto& ld : P.loads) {
+        const std::string x = "X[" + std::to_string(ld.second - tp->nparams) + "]";
+        os << "  R[" << ld.first << "] = " << (ball ? "cb_ddpt(" + x + ")" : x) << ";\n";
+    }
+    os << "  " << pre << "_interpret(" << fname << "_code, " << P.ninstr << ", " << fname << "_const, R, PB);\n";
+    for (int r = 0; r < nout; r++) {
+        if (P.out_operand[r] == INT32_MIN) os << "  out[" << r << "] = " << (ball ? "cb_pt(mkc(0.0, 0.0))" : "ic_zero()") << ";\n";
+        else os << "  out[" << r << "] = " << pre << "_fetch(" << fname << "_const, R, PB, " << P.out_operand[r] << ");\n";
+    }
+    os << "}\n";
+}
+
+bool generate_certify_source(const hcg_system* F, int n, std::string* src, std::string* err) {
+    if (!F || !F->eval || !F->eval_jac) { *err = "NULL system"; return false; }
+    if (n < 1 || n > HCG_MAX_N) { *err = "system size out of range (1.." + std::to_string(HCG_MAX_N) + ")"; return false; }
+    if (!validate_tape(F->eval_jac, n, err) || !validate_tape(F->eval, n, err)) return false;
+    if (F->eval_jac->nU == 0) { *err = "eval_jac tape has no Jacobian assignments"; return false; }
+    if (F->eval_jac->nparams != F->eval->nparams) { *err = "eval/eval_jac parameter counts differ"; return false; }
+    const int np = F->eval->nparams;
+    const ParamMode pm = np == 0 ? PM_NONE : (F->fixed_params ? PM_FIXED : PM_DYNAMIC);
+    const double* fixed = F->fixed_params;
+    std::ostringstream os;
+    os << "// generated by hcgpu codegen (certifier)\n#include \"hcg_arith.h\"\n#include \"hcg_interval.h\"\n";
+    os << "#define HCG_N " << n << "\n#define HCG_NPAR " << (pm == PM_DYNAMIC ? np : 0) << "\n";
+    long f;
+    gen_eval(os, F->eval_jac, "certF_eval_jac", true, false, n, pm, fixed, &f);
+    gen_dd_interp(os, F->eval, "certF_eval_dd", n, pm, fixed, &f);
+    gen_cert_interp(os, F->eval, "cert_ieval", false, n, pm, fixed, false);
+    gen_cert_interp(os, F->eval_jac, "cert_ijac", true, n, pm, fixed, false);
+    gen_cert_interp(os, F->eval, "cert_beval", false, n, pm, fixed, true);
+    gen_cert_interp(os, F->eval_jac, "cert_bjac", true, n, pm, fixed, true);
+    const std::string pa = pm == PM_DYNAMIC ? "P, P, mkc(0.0, 0.0), " : "";
+    os << "HD void cert_eval_jac(const cplx* x, const double* P, cplx* u, cplx* U) { certF_eval_jac(x, " << pa << "u, U); }\n";
+    os << "HD void cert_eval_dd(const cdd* x, const double* P, cdd* u) { certF_eval_dd(x, " << pa << "u); }\n";
+    os << "#include \"hcg_certify.h\"\n";
     *src = os.str();
     return true;
 }

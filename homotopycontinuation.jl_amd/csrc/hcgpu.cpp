@@ -75,6 +75,9 @@ struct hcg_program {
     // multi-system programs (hcg_program_create_multi): per-system size, kind, parameters
     int nsys = 0;
     std::vector<int> ns, kinds, npars;
+    // certifier programs (hcg_certifier_create): the kernel is hcg_certify_kernel
+    bool certifier = false;
+    int cert_npar = 0;  // parameters given per hcg_certify call (0 when fixed or absent)
 };
 
 extern "C" const char* hcg_last_error(void) { return g_err.c_str(); }
@@ -195,9 +198,28 @@ static std::vector<std::string> rtc_options() {
     return opts;
 }
 
+// Texts of the device headers the source includes, directly or through other headers.
+static std::string included_headers(const std::string& src) {
+    const size_t nh = sizeof(k_hdr_srcs) / sizeof(k_hdr_srcs[0]);
+    std::vector<char> used(nh, 0);
+    std::vector<std::string> todo = {src};
+    while (!todo.empty()) {
+        const std::string t = todo.back();
+        todo.pop_back();
+        for (size_t h = 0; h < nh; h++)
+            if (!used[h] && t.find(std::string("#include \"") + k_hdr_names[h] + "\"") != std::string::npos) {
+                used[h] = 1;
+                todo.push_back(k_hdr_srcs[h]);
+            }
+    }
+    std::string out;
+    for (size_t h = 0; h < nh; h++)
+        if (used[h]) out += k_hdr_srcs[h];
+    return out;
+}
+
 static std::string cache_key(const std::string& src) {
-    std::string k = src;
-    for (size_t h = 0; h < sizeof(k_hdr_srcs) / sizeof(k_hdr_srcs[0]); h++) k += k_hdr_srcs[h];
+    std::string k = src + included_headers(src);
     for (auto& o : rtc_options()) k += "\n" + o;
     int maj = 0, min = 0;
     hiprtcVersion(&maj, &min);
@@ -360,11 +382,13 @@ struct DevBuf {
 }  // namespace
 
 #include "device/hcg_kargs.h"
+#include "device/hcg_cert_kargs.h"
 
 static int validate_batch_args(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
                                bool host) {
     if (!ctx || !prog || !b || !opts) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
+    if (prog->certifier) return fail(HCG_ERR_INVALID_ARGUMENT, "certifier program: use hcg_certify");
     if (prog->nsys) return fail(HCG_ERR_INVALID_ARGUMENT, "multi-system program: use hcg_track_multi");
     const long np = b->npaths;
     if (np < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "negative path count");
@@ -738,5 +762,163 @@ extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_
         return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_multi: %s", ex.what());
     } catch (...) {
         return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_multi: unexpected error");
+    }
+}
+
+// ------------------------------ certification ------------------------------
+static int certifier_source(const hcg_system* F, int32_t n, std::string* src) {
+    if (!F) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL system");
+    std::string err;
+    if (!hcg::generate_certify_source(F, n, src, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    return HCG_OK;
+}
+
+static int hcg_certifier_create_impl(hcg_ctx* ctx, const hcg_system* F, int32_t n, hcg_program** out) {
+    if (!ctx || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    hcg_program* p = new hcg_program();
+    p->ctx = ctx;
+    int rc = certifier_source(F, n, &p->source);
+    if (rc) { delete p; return rc; }
+    p->n = n;
+    p->certifier = true;
+    p->cert_npar = F->fixed_params ? 0 : F->eval->nparams;
+    std::vector<char> code;
+    rc = rtc_compile(p->source, &code);
+    if (rc) { delete p; return rc; }
+    hipError_t e = hipModuleLoadData(&p->module, code.data());
+    if (e != hipSuccess) { delete p; return fail(HCG_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e)); }
+    e = hipModuleGetFunction(&p->kernel, p->module, "hcg_certify_kernel");
+    if (e != hipSuccess) { hipModuleUnload(p->module); delete p; return fail(HCG_ERR_HIP, "hipModuleGetFunction: %s", hipGetErrorString(e)); }
+    *out = p;
+    return HCG_OK;
+}
+
+static int hcg_certify_impl(hcg_ctx* ctx, const hcg_program* prog, int32_t nsol, const double* cand,
+                            int32_t ngroups, const double* params, const double* params_ival,
+                            const double* params_ball, const int32_t* group_of, int32_t refine, int32_t extended,
+                            int32_t real_system, hcg_certificate* out, double* box1, double* box0, double* refined) {
+    if (!ctx || !prog) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (prog->ctx != ctx) return fail(HCG_ERR_INVALID_ARGUMENT, "program belongs to another context");
+    if (!prog->certifier) return fail(HCG_ERR_INVALID_ARGUMENT, "not a certifier program (use hcg_certifier_create)");
+    if (nsol < 0) return fail(HCG_ERR_INVALID_ARGUMENT, "negative candidate count");
+    const int n = prog->n, np = prog->cert_npar;
+    if ((long)nsol > (1L << 28) / n) return fail(HCG_ERR_INVALID_ARGUMENT, "too many candidates");
+    if (nsol == 0) return HCG_OK;
+    if (!cand || !out || !box1 || !box0 || !refined) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL input or output");
+    if (np > 0) {
+        if (!params) return fail(HCG_ERR_INVALID_ARGUMENT, "the system has %d parameters: params is NULL", np);
+        if (ngroups < 1 || (long)ngroups * np > (1L << 24))
+            return fail(HCG_ERR_INVALID_ARGUMENT, "ngroups must be in 1..%ld", (1L << 24) / np);
+        if (group_of)
+            for (int32_t k = 0; k < nsol; k++)
+                if (group_of[k] < 0 || group_of[k] >= ngroups)
+                    return fail(HCG_ERR_INVALID_ARGUMENT, "group_of_candidate[%d] = %d out of range", k, group_of[k]);
+    } else {
+        ngroups = 0;
+        group_of = nullptr;
+    }
+    // per group and parameter: the FP64 box and the double-double ball (default: the point)
+    std::vector<double> ival(4 * (size_t)np * ngroups), ball(6 * (size_t)np * ngroups);
+    for (long q = 0; q < (long)np * ngroups; q++) {
+        const double re = params[2 * q], im = params[2 * q + 1];
+        double b[4] = {re, re, im, im};
+        if (params_ival) for (int t = 0; t < 4; t++) b[t] = params_ival[4 * q + t];
+        if (!(b[0] <= b[1]) || !(b[2] <= b[3]))
+            return fail(HCG_ERR_INVALID_ARGUMENT, "parameter %ld: invalid interval", q);
+        if (!(b[0] <= re && re <= b[1] && b[2] <= im && im <= b[3]))
+            return fail(HCG_ERR_INVALID_ARGUMENT, "parameter %ld: the point is not in its interval", q);
+        double c[6] = {re, 0.0, 0.0, im, 0.0, 0.0};
+        if (params_ball) for (int t = 0; t < 6; t++) c[t] = params_ball[6 * q + t];
+        if (!(c[2] >= 0.0) || !(c[5] >= 0.0)) return fail(HCG_ERR_INVALID_ARGUMENT, "parameter %ld: invalid ball radius", q);
+        for (int t = 0; t < 4; t++) ival[4 * q + t] = b[t];
+        for (int t = 0; t < 6; t++) ball[6 * q + t] = c[t];
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t zc = sizeof(double) * 2 * n * (size_t)nsol, zb = 2 * zc;
+    DevBuf d_c, d_p, d_ip, d_bp, d_g, d_out, d_b1, d_b0, d_r;
+    HIPCHK(hipMalloc(&d_c.p, zc));
+    HIPCHK(hipMalloc(&d_out.p, sizeof(hcg_certificate) * nsol));
+    HIPCHK(hipMalloc(&d_b1.p, zb));
+    HIPCHK(hipMalloc(&d_b0.p, zb));
+    HIPCHK(hipMalloc(&d_r.p, zc));
+    HIPCHK(hipMemcpyAsync(d_c.p, cand, zc, hipMemcpyHostToDevice, ctx->stream));
+    if (np > 0) {
+        const size_t m = (size_t)np * ngroups;
+        HIPCHK(hipMalloc(&d_p.p, sizeof(double) * 2 * m));
+        HIPCHK(hipMalloc(&d_ip.p, sizeof(double) * 4 * m));
+        HIPCHK(hipMalloc(&d_bp.p, sizeof(double) * 6 * m));
+        HIPCHK(hipMemcpyAsync(d_p.p, params, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_ip.p, ival.data(), sizeof(double) * 4 * m, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(d_bp.p, ball.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice, ctx->stream));
+        if (group_of) {
+            HIPCHK(hipMalloc(&d_g.p, sizeof(int32_t) * nsol));
+            HIPCHK(hipMemcpyAsync(d_g.p, group_of, sizeof(int32_t) * nsol, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    hcg_cert_kargs ka;
+    std::memset(&ka, 0, sizeof ka);
+    ka.nsol = nsol;
+    ka.refine = refine ? 1 : 0;
+    ka.real_system = real_system ? 1 : 0;
+    ka.extended = extended ? 1 : 0;
+    ka.candidates = (const double*)d_c.p;
+    ka.params = (const double*)d_p.p;
+    ka.params_ival = (const double*)d_ip.p;
+    ka.params_ball = (const double*)d_bp.p;
+    ka.group_of = (const int32_t*)d_g.p;
+    ka.out = (hcg_certificate*)d_out.p;
+    ka.box1 = (double*)d_b1.p;
+    ka.box0 = (double*)d_b0.p;
+    ka.refined = (double*)d_r.p;
+    const unsigned grid = (unsigned)(((long)nsol + 63) / 64);  // one lane per candidate, 64 per workgroup
+    void* args[] = {&ka};
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipModuleLaunchKernel(prog->kernel, grid, 1, 1, 64, 1, 1, 0, ctx->stream, args, nullptr));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, d_out.p, sizeof(hcg_certificate) * nsol, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(box1, d_b1.p, zb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(box0, d_b0.p, zb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(refined, d_r.p, zc, hipMemcpyDeviceToHost, ctx->stream));
+    return finish_launch(ctx);
+}
+
+extern "C" int hcg_certifier_compile_only(const hcg_system* F, int32_t n, size_t* code_size) {
+    try {
+        std::string src;
+        int rc = certifier_source(F, n, &src);
+        if (rc) return rc;
+        std::vector<char> code;
+        rc = rtc_compile(src, &code);
+        if (rc) return rc;
+        if (code_size) *code_size = code.size();
+        return HCG_OK;
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certifier_compile_only: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certifier_compile_only: unexpected error");
+    }
+}
+extern "C" int hcg_certifier_create(hcg_ctx* ctx, const hcg_system* F, int32_t n, hcg_program** out) {
+    try {
+        return hcg_certifier_create_impl(ctx, F, n, out);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certifier_create: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certifier_create: unexpected error");
+    }
+}
+extern "C" int hcg_certify(hcg_ctx* ctx, const hcg_program* cert, int32_t nsol, const double* candidates,
+                           int32_t ngroups, const double* params, const double* params_ival,
+                           const double* params_ball, const int32_t* group_of_candidate, int32_t refine,
+                           int32_t extended, int32_t real_system, hcg_certificate* out, double* box1, double* box0,
+                           double* refined) {
+    try {
+        return hcg_certify_impl(ctx, cert, nsol, candidates, ngroups, params, params_ival, params_ball,
+                                group_of_candidate, refine, extended, real_system, out, box1, box0, refined);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certify: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_certify: unexpected error");
     }
 }

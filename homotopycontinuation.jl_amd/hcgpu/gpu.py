@@ -40,6 +40,14 @@ class GpuContext:
             self._programs[key] = p
         return p
 
+    def certifier(self, system, fixed_params=None) -> "Certifier":
+        key = ("certifier", id(system), None if fixed_params is None else tuple(np.asarray(fixed_params).ravel()))
+        c = self._programs.get(key)
+        if c is None:
+            c = Certifier(self, system, fixed_params)
+            self._programs[key] = c
+        return c
+
     def last_kernel_ms(self) -> float:
         ms = ctypes.c_double()
         check(lib().hcg_last_kernel_stats(self._h, ctypes.byref(ms), None, None))
@@ -269,3 +277,78 @@ def track_multi(program: MultiProgram, starts_list, opts: Optional[hcg_options] 
         off += np_
         offx += np_ * n
     return outs
+
+
+_CERT_DTYPE = np.dtype([(f, np.int32 if t is ctypes.c_int32 else np.float64) for f, t in abi.hcg_certificate._fields_],
+                       align=True)
+assert _CERT_DTYPE.itemsize == ctypes.sizeof(abi.hcg_certificate)
+
+
+class CertifyOutput:
+    """Per-candidate arrays of one hcg_certify call: the hcg_certificate fields, the boxes
+    x₁ (box1) and x₀ (box0) as (nsol, n, 4) arrays of (re lo, re hi, im lo, im hi), and the
+    refined candidates."""
+
+    def __init__(self, rec, box1, box0, refined):
+        self.records = rec
+        for f in rec.dtype.names:
+            setattr(self, f, rec[f])
+        self.box1, self.box0, self.refined = box1, box0, refined
+
+
+class Certifier:
+    """hcg_certifier_create: a square system's point, double-double and interval
+    evaluations plus the Krawczyk kernel, compiled with hipRTC."""
+
+    def __init__(self, ctx: GpuContext, system, fixed_params=None):
+        from .homotopies import _SysTapes
+        self.ctx = ctx
+        self.system = system
+        self.n = system.nvars
+        if len(system) != self.n:
+            raise ValueError("only square systems can be certified")
+        self.nparams = 0 if fixed_params is not None else system.nparams
+        self._tapes = _SysTapes(system, fixed_params if system.nparams else None)
+        self._struct = self._tapes.struct()
+        self._h = ctypes.c_void_p()
+        check(lib().hcg_certifier_create(ctx.handle, ctypes.byref(self._struct), self.n, ctypes.byref(self._h)))
+
+    def certify(self, candidates, params=None, params_ival=None, params_ball=None, refine=True, extended=True,
+                real_system=True, group_of=None) -> CertifyOutput:
+        """params (and params_ival, params_ball): one parameter group (nparams, ...) or several
+        (ngroups, nparams, ...); group_of: the group of each candidate (default group 0)."""
+        X = np.ascontiguousarray(np.asarray(candidates, dtype=np.complex128).reshape(-1, self.n))
+        nsol = len(X)
+        rec = np.zeros(nsol, dtype=_CERT_DTYPE)
+        box1 = np.zeros((nsol, self.n, 4))
+        box0 = np.zeros((nsol, self.n, 4))
+        ref = np.zeros((nsol, self.n), dtype=np.complex128)
+        P = IP = BP = G = None
+        ngroups = 0
+        if self.nparams:
+            if params is None:
+                raise ValueError("the system has parameters: pass params")
+            P = np.ascontiguousarray(np.asarray(params, dtype=np.complex128).reshape(-1, self.nparams))
+            ngroups = len(P)
+            if params_ival is not None:
+                IP = np.ascontiguousarray(np.asarray(params_ival, dtype=np.float64).reshape(ngroups, self.nparams, 4))
+            if params_ball is not None:
+                BP = np.ascontiguousarray(np.asarray(params_ball, dtype=np.float64).reshape(ngroups, self.nparams, 6))
+            if group_of is not None:
+                G = np.ascontiguousarray(np.asarray(group_of, dtype=np.int32).reshape(nsol))
+        keep = (X, P, IP, BP, G)  # noqa: F841  (alive during the call)
+        opt = lambda a: _dp(a) if a is not None else None  # noqa: E731
+        check(lib().hcg_certify(self.ctx.handle, self._h, nsol, _dp(X), ngroups, opt(P), opt(IP), opt(BP),
+                                G.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if G is not None else None,
+                                int(refine), int(extended), int(real_system),
+                                rec.ctypes.data_as(ctypes.POINTER(abi.hcg_certificate)), _dp(box1), _dp(box0),
+                                _dp(ref)))
+        return CertifyOutput(rec, box1, box0, ref)
+
+    def source(self) -> str:
+        return lib().hcg_program_source(self._h).decode()
+
+    def close(self):
+        if self._h:
+            lib().hcg_program_destroy(self._h)
+            self._h = ctypes.c_void_p()

@@ -264,6 +264,51 @@ int hcg_mlp_train(hcg_ctx* ctx, float* params, int32_t n, const float* x, const 
                   const float* w, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                   float* final_loss);
 
+/* ---------------- certification ----------------
+ * Interval-arithmetic Krawczyk certification of approximate zeros of a square system
+ * F(x; p), one candidate per GPU lane.  Replaces the per-candidate loop of _certify
+ * (src/certification.jl:779-975) calling certify_solution (:1029-1114) with the FP64
+ * ε-inflation test ε_inflation_krawczyk (:1225-1287), in the interval arithmetic of
+ * src/interval_arithmetic.jl.  The reference retries candidates that fail in FP64 with
+ * Arb balls of growing precision (extended_prec_certify_solution, :1116-1218); here a
+ * failed candidate is retried once with the residual F(x̃) enclosed in double-double
+ * balls (about 106 bits), the rest of the test staying in FP64 intervals.  Candidates that
+ * fail both are reported as not certified.  Duplicate detection among certified boxes
+ * (DistinctSolutionCertificates, :300-427) is host work on the returned boxes. */
+typedef struct hcg_certificate {
+    int32_t certified;    /* the Krawczyk test holds: exactly one zero in the box x₀, and it lies in x₁ */
+    int32_t real;         /* that zero is real (conj(x₁) in the interior of x₀, F real on real points) */
+    int32_t complex_;     /* some coordinate of x₁ excludes the real axis: that zero is not real */
+    int32_t newton_code;  /* refinement: 0 success, 1 rejected, 2 max_iters, -1 not refined */
+    int32_t newton_iters;
+    int32_t prec;         /* 53: FP64 residual enclosure; 106: the double-double retry */
+    double contraction;   /* 1.41422 · max_i Σ_j mag(M_ij) for M = C·J(x₀) − I; certified needs < 0.7071 */
+    double inflation;     /* largest inflation radius ε_i of x₀ */
+} hcg_certificate;
+
+/* Compile a certifier for the system F (its eval_jac and eval tapes; F.fixed_params fixes
+ * the parameters, else they are given per hcg_certify call).  Counterpart of
+ * CertificationCache(F) (src/certification.jl:631-671), which builds the IComplexF64
+ * interpreters of F and of its Jacobian. */
+int hcg_certifier_create(hcg_ctx* ctx, const hcg_system* F, int32_t n, hcg_program** out);
+int hcg_certifier_compile_only(const hcg_system* F, int32_t n, size_t* code_size);
+/* Certify nsol candidates (nsol * n complex).  Parameters come in ngroups groups, candidate
+ * k belonging to group group_of_candidate[k] (NULL: all to group 0): params holds ngroups
+ * parameter points (nparams complex each) used by the Newton refinement; params_ival their
+ * enclosures (nparams x (re lo, re hi, im lo, im hi) each), NULL = the points themselves;
+ * params_ball double-double enclosures for the extended retry (nparams x (re hi, re lo,
+ * re radius, im hi, im lo, im radius) each), NULL = the points.  A system without free
+ * parameters takes ngroups = 0 and NULL for all four.  refine = 1 refines each candidate by
+ * Newton's method first (the reference's default); extended = 1 enables the double-double
+ * retry.  real_system = 1 if F(·; p) maps real points to real values for every group
+ * (is_real(F), src/model_kit/abstract_system_homotopy.jl:85-89, decided by the caller).
+ * Outputs: out[nsol]; box1 / box0: nsol * n * 4 doubles (re lo, re hi, im lo, im hi) of x₁
+ * and x₀; refined: nsol * n complex. */
+int hcg_certify(hcg_ctx* ctx, const hcg_program* cert, int32_t nsol, const double* candidates, int32_t ngroups,
+                const double* params, const double* params_ival, const double* params_ball,
+                const int32_t* group_of_candidate, int32_t refine, int32_t extended, int32_t real_system,
+                hcg_certificate* out, double* box1, double* box0, double* refined);
+
 const char* hcg_last_error(void);
 
 #ifdef __cplusplus

@@ -62,6 +62,13 @@ def lib():
         L.orc_sizeof.argtypes = [ctypes.c_int]
         L.orc_sizeof.restype = ctypes.c_size_t
         L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_certify.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, P(D), ctypes.c_int, P(D), P(D), P(D),
+                                  P(ctypes.c_int32), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_void_p, P(D), P(D), P(D)]
+        L.orc_ival_op.argtypes = [ctypes.c_int, P(D), P(D), P(D), P(D)]
+        L.orc_ival_op.restype = None
+        L.orc_ball_op.argtypes = [ctypes.c_int, P(D), P(D), P(D)]
+        L.orc_ball_op.restype = None
         _lib = L
     return _lib
 
@@ -271,3 +278,61 @@ def segment_run(start, target, ops):
         rows.append(dict(s=r[0], s1=r[1], ds=r[2], t=complex(r[3], r[4]), t1=complex(r[5], r[6]),
                          dt=complex(r[7], r[8]), abs_delta=r[9], forward=bool(r[10]), done=bool(r[11]), dist=r[12]))
     return rows
+
+
+class CertifyOutput:
+    """Per-candidate arrays of one certification batch (hcg_certificate fields, the boxes
+    x₁ and x₀ as (nsol, n, 4) arrays of (re lo, re hi, im lo, im hi), refined candidates)."""
+
+    def __init__(self, rec, box1, box0, refined):
+        self.records = rec
+        for f in rec.dtype.names:
+            setattr(self, f, rec[f])
+        self.box1, self.box0, self.refined = box1, box0, refined
+
+
+def certify(system, n, candidates, params=None, params_ival=None, params_ball=None, refine=True, extended=True,
+            real_system=True, nthreads=0, group_of=None):
+    """The oracle's certification of `candidates` (nsol x n) for an hcg_system struct.
+    params / params_ival / params_ball: one group (nparams, ...) or several (ngroups, nparams,
+    ...); group_of: the group of each candidate (default group 0)."""
+    from hcgpu.abi import hcg_certificate
+    X = _c(candidates).reshape(-1, n)
+    nsol = len(X)
+    rec = np.zeros(nsol, dtype=np.dtype([(f, np.int32 if t is ctypes.c_int32 else np.float64)
+                                         for f, t in hcg_certificate._fields_], align=True))
+    box1 = np.zeros((nsol, n, 4))
+    box0 = np.zeros((nsol, n, 4))
+    ref = np.zeros((nsol, n), dtype=np.complex128)
+    P = _c(params) if params is not None else None
+    ngroups = 0 if P is None else (P.shape[0] if P.ndim == 2 else 1)
+    IP = np.ascontiguousarray(np.asarray(params_ival, dtype=np.float64)) if params_ival is not None else None
+    BP = np.ascontiguousarray(np.asarray(params_ball, dtype=np.float64)) if params_ball is not None else None
+    G = np.ascontiguousarray(np.asarray(group_of, dtype=np.int32)) if group_of is not None else None
+    keep = (X, P, IP, BP, G)  # noqa: F841  (alive during the call)
+    nthreads = nthreads or min(os.cpu_count() or 1, 16)
+    opt = lambda a: _dp(a) if a is not None else None  # noqa: E731
+    rc = lib().orc_certify(ctypes.cast(ctypes.pointer(system), ctypes.c_void_p), n, nsol, _dp(X), ngroups, opt(P),
+                           opt(IP), opt(BP), G.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if G is not None else None,
+                           int(refine), int(extended), int(real_system), nthreads,
+                           rec.ctypes.data_as(ctypes.c_void_p), _dp(box1), _dp(box0), _dp(ref))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return CertifyOutput(rec, box1, box0, ref)
+
+
+def ival_op(op, a, b=(0.0, 0.0), c=(0.0, 0.0)):
+    """One interval operation of the oracle (see orc_ival_op): returns (lo, hi)."""
+    A, B, Cc = (np.asarray(v, dtype=np.float64) for v in (a, b, c))
+    out = np.zeros(2)
+    lib().orc_ival_op(op, _dp(A), _dp(B), _dp(Cc), _dp(out))
+    return float(out[0]), float(out[1])
+
+
+def ball_op(op, a, b):
+    """One double-double ball operation of the oracle (0 add, 1 sub, 2 mul) on balls given as
+    (hi, lo, radius); returns (hi, lo, radius)."""
+    A, B = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_ball_op(op, _dp(A), _dp(B), _dp(out))
+    return tuple(float(v) for v in out)

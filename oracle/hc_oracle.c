@@ -20,6 +20,8 @@
  *   valuation                                        src/valuation.jl:38-228
  *   endgame tracker + PathResult                     src/endgame_tracker.jl:260-911
  *   threaded batch driver                            src/solve.jl:628-709
+ *   certification (interval Krawczyk)                src/certification.jl:1029-1394,
+ *                                                    src/interval_arithmetic.jl
  *
  * Pinning (see tests/): DD arithmetic against mpmath; SLP eval/Jacobian/Taylor
  * against closed-form polynomial evaluation and exact series composition; LU/solve
@@ -2268,4 +2270,612 @@ int orc_track_plain_valuation(const orc_homotopy_desc* hd, const orc_options* o,
     free(x);
     tracker_destroy(&T);
     return code;
+}
+
+/* ============================ certification ============================
+ * Krawczyk certification of approximate zeros (src/certification.jl:1029-1114
+ * certify_solution, :1225-1287 ε_inflation_krawczyk) in the FP64 interval arithmetic of
+ * src/interval_arithmetic.jl: every bound is the round-to-nearest result moved one ulp
+ * outward (round_down = prevfloat, round_up = nextfloat, :57-61), products and quotients
+ * by the case analysis of :118-234.  Interval muladd uses FMA (one rounding per bound).
+ * Pinned by exact rational enclosure checks of every interval operation, the reference's
+ * certification known answers (test/certification_test.jl) and the 3264 real conics. */
+typedef struct { double lo, hi; } ival;
+typedef struct { ival re, im; } icx;
+
+static inline double r_up(double x) { return nextafter(x, INFINITY); }
+static inline double r_dn(double x) { return nextafter(x, -INFINITY); }
+static inline ival IV(double lo, double hi) { ival r = {lo, hi}; return r; }
+static inline ival iround(double lo, double hi) { return IV(r_dn(lo), r_up(hi)); }
+static inline int i_iszero(ival a) { return a.lo == 0 && a.hi == 0; }
+static inline int i_in(double x, ival a) { return a.lo <= x && x <= a.hi; }
+static inline double jmin_(double a, double b) { if (isnan(a) || isnan(b)) return NAN; return a < b ? a : b; }
+static inline double i_mag(ival a) { return jmax(fabs(a.lo), fabs(a.hi)); }
+static inline double i_mig(ival a) { return i_in(0.0, a) ? 0.0 : jmin_(fabs(a.lo), fabs(a.hi)); }
+static inline double min_fast_(double x, double y) { return y < x ? y : x; }
+static inline int i_interior(ival a, ival b) { return a.lo > b.lo && a.hi < b.hi; }
+
+static ival i_add(ival a, ival b) { return iround(a.lo + b.lo, a.hi + b.hi); }
+static ival i_add_r(ival a, double b) { return iround(a.lo + b, a.hi + b); }
+static ival i_sub(ival a, ival b) { return iround(a.lo - b.hi, a.hi - b.lo); }
+static ival i_sub_r(ival a, double b) { return iround(a.lo - b, a.hi - b); }
+static ival i_neg(ival a) { return IV(-a.hi, -a.lo); }
+static ival i_mul_r(double x, ival a) {
+    if (i_iszero(a) || x == 0) return IV(0, 0);
+    if (x >= 0.0) return iround(a.lo * x, a.hi * x);
+    return iround(a.hi * x, a.lo * x);
+}
+static ival i_mul(ival a, ival b) {
+    if (b.lo >= 0) {
+        if (a.lo >= 0) return iround(a.lo * b.lo, a.hi * b.hi);
+        if (a.hi <= 0) return iround(a.lo * b.hi, a.hi * b.lo);
+        return iround(a.lo * b.hi, a.hi * b.hi);
+    } else if (b.hi <= 0) {
+        if (a.lo >= 0) return iround(a.hi * b.lo, a.lo * b.hi);
+        if (a.hi <= 0) return iround(a.hi * b.hi, a.lo * b.lo);
+        return iround(a.hi * b.lo, a.lo * b.lo);
+    } else {
+        if (a.lo > 0) return iround(a.hi * b.lo, a.hi * b.hi);
+        if (a.hi < 0) return iround(a.lo * b.hi, a.lo * b.lo);
+        return iround(jmin_(a.lo * b.hi, a.hi * b.lo), jmax(a.lo * b.lo, a.hi * b.hi));
+    }
+}
+static ival i_muladd(ival a, ival b, ival c) {
+    double lo = min_fast_(min_fast_(min_fast_(fma(a.lo, b.lo, c.lo), fma(a.lo, b.hi, c.lo)), fma(a.hi, b.lo, c.lo)),
+                          fma(a.hi, b.hi, c.lo));
+    double hi = max_fast(max_fast(max_fast(fma(a.lo, b.lo, c.hi), fma(a.lo, b.hi, c.hi)), fma(a.hi, b.lo, c.hi)),
+                         fma(a.hi, b.hi, c.hi));
+    return IV(r_dn(lo), r_up(hi));
+}
+static ival i_muladd_r(double a, ival b, ival c) {
+    double lo = min_fast_(fma(a, b.lo, c.lo), fma(a, b.hi, c.lo));
+    double hi = max_fast(fma(a, b.lo, c.hi), fma(a, b.hi, c.hi));
+    return IV(r_dn(lo), r_up(hi));
+}
+static ival i_div(ival a, ival b) {
+    if (i_iszero(a)) return IV(0, 0);
+    if (b.lo > 0) {
+        if (a.lo >= 0) return iround(a.lo / b.hi, a.hi / b.lo);
+        if (a.hi <= 0) return iround(a.lo / b.lo, a.hi / b.hi);
+        return iround(a.lo / b.lo, a.hi / b.lo);
+    } else if (b.hi < 0) {
+        if (a.lo >= 0) return iround(a.hi / b.hi, a.lo / b.lo);
+        if (a.hi <= 0) return iround(a.hi / b.lo, a.lo / b.hi);
+        return iround(a.hi / b.hi, a.lo / b.hi);
+    }
+    return IV(NAN, NAN);
+}
+static ival i_inv(ival a) {
+    if (i_in(0.0, a)) return IV(NAN, NAN);
+    return iround(1.0 / a.hi, 1.0 / a.lo);
+}
+static ival i_sqr(ival a) {
+    if (a.lo >= 0) return iround(a.lo * a.lo, a.hi * a.hi);
+    if (a.hi <= 0) return iround(a.hi * a.hi, a.lo * a.lo);
+    double g = i_mig(a), m = i_mag(a);
+    return iround(g * g, m * m);
+}
+
+static inline icx ICX(ival re, ival im) { icx z = {re, im}; return z; }
+static inline icx ic_point(cplx z) { return ICX(IV(z.re, z.re), IV(z.im, z.im)); }
+static inline icx ic_zero_(void) { return ICX(IV(0, 0), IV(0, 0)); }
+static inline double icx_mag(icx z) { return jmax(i_mag(z.re), i_mag(z.im)); }
+static icx icx_add(icx a, icx b) { return ICX(i_add(a.re, b.re), i_add(a.im, b.im)); }
+static icx icx_sub(icx a, icx b) { return ICX(i_sub(a.re, b.re), i_sub(a.im, b.im)); }
+static icx icx_neg(icx a) { return ICX(i_neg(a.re), i_neg(a.im)); }
+static icx icx_mul(icx z, icx w) {
+    return ICX(i_sub(i_mul(z.re, w.re), i_mul(z.im, w.im)), i_add(i_mul(z.re, w.im), i_mul(z.im, w.re)));
+}
+static icx icx_inv(icx b) {
+    ival d = i_add(i_sqr(b.re), i_sqr(b.im));
+    return ICX(i_div(b.re, d), i_div(i_neg(b.im), d));
+}
+static icx icx_div(icx a, icx b) {
+    ival d = i_add(i_sqr(b.re), i_sqr(b.im));
+    return ICX(i_div(i_add(i_mul(a.re, b.re), i_mul(a.im, b.im)), d), i_div(i_sub(i_mul(a.im, b.re), i_mul(a.re, b.im)), d));
+}
+/* muladd(z::Complex, w::IComplex, x::IComplex) and muladd on IComplex, :372-375 */
+static icx icx_muladd_pt(cplx z, icx w, icx x) {
+    return ICX(i_sub(i_muladd_r(z.re, w.re, x.re), i_mul_r(z.im, w.im)), i_muladd_r(z.re, w.im, i_muladd_r(z.im, w.re, x.im)));
+}
+static icx icx_muladd(icx z, icx w, icx x) {
+    return ICX(i_sub(i_muladd(z.re, w.re, x.re), i_mul(z.im, w.im)), i_muladd(z.re, w.im, i_muladd(z.im, w.re, x.im)));
+}
+/* Base.power_by_squaring on IComplex */
+static icx icx_pbs(icx x, long p) {
+    if (p == 1) return x;
+    if (p == 0) return ICX(IV(1, 1), IV(0, 0));
+    if (p == 2) return icx_mul(x, x);
+    int t = __builtin_ctzl((unsigned long)p) + 1;
+    p >>= t;
+    while (--t > 0) x = icx_mul(x, x);
+    icx y = x;
+    while (p > 0) {
+        t = __builtin_ctzl((unsigned long)p) + 1;
+        p >>= t;
+        while (--t >= 0) x = icx_mul(x, x);
+        y = icx_mul(y, x);
+    }
+    return y;
+}
+
+/* The generic operations (src/model_kit/operations.jl:183-248) on IComplex: one interval
+ * execution of the tape. */
+static void tape_run_ival(const orc_tape* tp, icx* T) {
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        if (op == OP_STOP) break;
+        icx a = T[I[0] - 1], b, c, d, r;
+        switch (op) {
+        case OP_CB: r = icx_mul(icx_mul(a, a), a); break;
+        case OP_INV: r = icx_inv(a); break;
+        case OP_INV_NOT_ZERO: r = (i_iszero(a.re) && i_iszero(a.im)) ? a : icx_inv(a); break;
+        case OP_INVSQR: b = icx_inv(a); r = icx_mul(b, b); break;
+        case OP_NEG: r = icx_neg(a); break;
+        case OP_SQR: r = icx_mul(a, a); break;
+        case OP_IDENTITY: r = a; break;
+        case OP_ADD: r = icx_add(a, T[I[1] - 1]); break;
+        case OP_SUB: r = icx_sub(a, T[I[1] - 1]); break;
+        case OP_MUL: r = icx_mul(a, T[I[1] - 1]); break;
+        case OP_DIV: r = icx_div(a, T[I[1] - 1]); break;
+        case OP_POW_INT: r = I[1] > 0 ? icx_pbs(a, I[1]) : icx_inv(icx_pbs(a, -(long)I[1])); break;
+        case OP_ADD3: b = T[I[1] - 1]; c = T[I[2] - 1]; r = icx_add(icx_add(a, b), c); break;
+        case OP_MUL3: b = T[I[1] - 1]; c = T[I[2] - 1]; r = icx_mul(icx_mul(a, b), c); break;
+        case OP_MULADD: b = T[I[1] - 1]; c = T[I[2] - 1]; r = icx_add(icx_mul(a, b), c); break;
+        case OP_MULSUB: b = T[I[1] - 1]; c = T[I[2] - 1]; r = icx_sub(icx_mul(a, b), c); break;
+        case OP_SUBMUL: b = T[I[1] - 1]; c = T[I[2] - 1]; r = icx_sub(c, icx_mul(a, b)); break;
+        case OP_ADD4: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = icx_add(icx_add(icx_add(a, b), c), d); break;
+        case OP_MUL4: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = icx_mul(icx_mul(icx_mul(a, b), c), d); break;
+        case OP_MULMULADD: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = icx_add(icx_mul(a, b), icx_mul(c, d)); break;
+        case OP_MULMULSUB: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = icx_sub(icx_mul(a, b), icx_mul(c, d)); break;
+        default: r = ICX(IV(NAN, NAN), IV(NAN, NAN));
+        }
+        T[I[5] - 1] = r;
+    }
+}
+/* F (jac = 0: u) or ∂F/∂x (jac = 1: U, column-major) on the box X; parameters from the
+ * packed box IP (re lo, re hi, im lo, im hi) or, for a fixed-parameter system, points. */
+static void sys_eval_ival(const orc_tape* tp, const double* fixed, const icx* X, const double* IP, int n, int jac,
+                          icx* T, icx* out) {
+    for (int k = 0; k < tp->nconst; k++) T[k] = ic_point(C(tp->constants[2 * k], tp->constants[2 * k + 1]));
+    for (int k = 0; k < tp->nparams; k++)
+        T[tp->params_first - 1 + k] = fixed ? ic_point(C(fixed[2 * k], fixed[2 * k + 1]))
+                                            : ICX(IV(IP[4 * k], IP[4 * k + 1]), IV(IP[4 * k + 2], IP[4 * k + 3]));
+    for (int k = 0; k < tp->nvars; k++) T[tp->vars_first - 1 + k] = X[k];
+    tape_run_ival(tp, T);
+    int m = jac ? n * n : n, na = jac ? tp->nU : tp->nu;
+    const int32_t* as = jac ? tp->U_assign : tp->u_assign;
+    for (int i = 0; i < m; i++) out[i] = ic_zero_();
+    for (int i = 0; i < na; i++) out[as[2 * i] - 1] = T[as[2 * i + 1] - 1];
+}
+
+/* Double-double balls for the extended retry: (m, r) holds every v with |v - m| <= r.
+ * Midpoint errors: the accurate double-double sum errs by less than 3u²/(1-4u) of the exact
+ * sum, the product (dd_mul, DWTimesDW1) by less than 7u² of the exact product (Joldes,
+ * Muller, Popescu, ACM TOMS 44(2), 2017); the radius adds 8u² and 16u² of the magnitudes
+ * plus 1e-300, rounded upward. */
+typedef struct { dd m; double r; } dball;
+typedef struct { dball re, im; } cball;
+#define BALL_U2 1.2325951644078310e-32
+static inline double dd_mag_(dd a) { return r_up(fabs(a.hi) + fabs(a.lo)); }
+static dd dd_add_acc(dd x, dd y) {
+    dd s = two_sum(x.hi, y.hi), t = two_sum(x.lo, y.lo);
+    double c = s.lo + t.hi;
+    dd v = quick_two_sum(s.hi, c);
+    double w = t.lo + v.lo;
+    return quick_two_sum(v.hi, w);
+}
+static inline dball DB(dd m, double r) { dball b = {m, r}; return b; }
+static dball db_add(dball a, dball b) {
+    dd m = dd_add_acc(a.m, b.m);
+    double e = r_up(r_up(8.0 * BALL_U2 * dd_mag_(m)) + 1e-300);
+    return DB(m, r_up(r_up(a.r + b.r) + e));
+}
+static dball db_neg(dball a) { return DB(DD(-a.m.hi, -a.m.lo), a.r); }
+static dball db_sub(dball a, dball b) { return db_add(a, db_neg(b)); }
+static dball db_mul(dball a, dball b) {
+    dd m = dd_mul(a.m, b.m);
+    double ma = dd_mag_(a.m), mb = dd_mag_(b.m);
+    double e = r_up(r_up(16.0 * BALL_U2 * r_up(ma * mb)) + 1e-300);
+    double r = r_up(r_up(r_up(ma * b.r) + r_up(mb * a.r)) + r_up(a.r * b.r));
+    return DB(m, r_up(r + e));
+}
+static ival db_ival(dball a) {
+    double t = a.m.hi + a.m.lo;
+    return IV(r_dn(r_dn(t) - a.r), r_up(r_up(t) + a.r));
+}
+static inline cball CB(dball re, dball im) { cball z = {re, im}; return z; }
+static cball cb_point(cplx z) { return CB(DB(DD(z.re, 0), 0), DB(DD(z.im, 0), 0)); }
+static cball cb_nan_(void) { return CB(DB(DD(NAN, 0), INFINITY), DB(DD(NAN, 0), INFINITY)); }
+static cball cb_add_(cball a, cball b) { return CB(db_add(a.re, b.re), db_add(a.im, b.im)); }
+static cball cb_sub_(cball a, cball b) { return CB(db_sub(a.re, b.re), db_sub(a.im, b.im)); }
+static cball cb_mul_(cball a, cball b) {
+    return CB(db_sub(db_mul(a.re, b.re), db_mul(a.im, b.im)), db_add(db_mul(a.re, b.im), db_mul(a.im, b.re)));
+}
+static cball cb_pow_(cball x, long p) {
+    if (p < 1) return cb_nan_();
+    if (p == 1) return x;
+    cball y = x;
+    int have = 0;
+    while (p > 0) {
+        if (p & 1) { y = have ? cb_mul_(y, x) : x; have = 1; }
+        p >>= 1;
+        if (p) x = cb_mul_(x, x);
+    }
+    return y;
+}
+/* F at the double-double point x in balls; parameters from B (re hi, re lo, re rad, im hi, im lo, im rad)
+ * or fixed points.  Quotients have no ball form: their results are unbounded. */
+static void sys_eval_ball(const orc_tape* tp, const double* fixed, const cdd* x, const double* B, int n, int jac,
+                          cball* T, cball* out) {
+    for (int k = 0; k < tp->nconst; k++) T[k] = cb_point(C(tp->constants[2 * k], tp->constants[2 * k + 1]));
+    for (int k = 0; k < tp->nparams; k++)
+        T[tp->params_first - 1 + k] = fixed ? cb_point(C(fixed[2 * k], fixed[2 * k + 1]))
+            : CB(DB(DD(B[6 * k], B[6 * k + 1]), B[6 * k + 2]), DB(DD(B[6 * k + 3], B[6 * k + 4]), B[6 * k + 5]));
+    for (int k = 0; k < tp->nvars; k++) T[tp->vars_first - 1 + k] = CB(DB(x[k].re, 0), DB(x[k].im, 0));
+    for (int i = 0; i < tp->ninstr; i++) {
+        const int32_t* I = tp->instr + 6 * i;
+        int op = I[4];
+        if (op == OP_STOP) break;
+        cball a = T[I[0] - 1], b, c, d, r;
+        switch (op) {
+        case OP_CB: r = cb_mul_(cb_mul_(a, a), a); break;
+        case OP_NEG: r = CB(db_neg(a.re), db_neg(a.im)); break;
+        case OP_SQR: r = cb_mul_(a, a); break;
+        case OP_IDENTITY: r = a; break;
+        case OP_ADD: r = cb_add_(a, T[I[1] - 1]); break;
+        case OP_SUB: r = cb_sub_(a, T[I[1] - 1]); break;
+        case OP_MUL: r = cb_mul_(a, T[I[1] - 1]); break;
+        case OP_POW_INT: r = cb_pow_(a, I[1]); break;
+        case OP_ADD3: b = T[I[1] - 1]; c = T[I[2] - 1]; r = cb_add_(cb_add_(a, b), c); break;
+        case OP_MUL3: b = T[I[1] - 1]; c = T[I[2] - 1]; r = cb_mul_(cb_mul_(a, b), c); break;
+        case OP_MULADD: b = T[I[1] - 1]; c = T[I[2] - 1]; r = cb_add_(cb_mul_(a, b), c); break;
+        case OP_MULSUB: b = T[I[1] - 1]; c = T[I[2] - 1]; r = cb_sub_(cb_mul_(a, b), c); break;
+        case OP_SUBMUL: b = T[I[1] - 1]; c = T[I[2] - 1]; r = cb_sub_(c, cb_mul_(a, b)); break;
+        case OP_ADD4: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = cb_add_(cb_add_(cb_add_(a, b), c), d); break;
+        case OP_MUL4: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = cb_mul_(cb_mul_(cb_mul_(a, b), c), d); break;
+        case OP_MULMULADD: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = cb_add_(cb_mul_(a, b), cb_mul_(c, d)); break;
+        case OP_MULMULSUB: b = T[I[1] - 1]; c = T[I[2] - 1]; d = T[I[3] - 1]; r = cb_sub_(cb_mul_(a, b), cb_mul_(c, d)); break;
+        default: r = cb_nan_();  /* inv, div: no ball form */
+        }
+        T[I[5] - 1] = r;
+    }
+    int m = jac ? n * n : n, na = jac ? tp->nU : tp->nu;
+    const int32_t* as = jac ? tp->U_assign : tp->u_assign;
+    for (int i = 0; i < m; i++) out[i] = cb_point(C(0, 0));
+    for (int i = 0; i < na; i++) out[as[2 * i] - 1] = T[as[2 * i + 1] - 1];
+}
+
+/* Double-double LU with partial pivoting (pivots on the rounded entries), its solve, and
+ * the inverse of the Jacobian at a double-double point rounded to FP64 (the role of
+ * approx_inv! on the Arb Jacobian, src/certification.jl:1384-1391). */
+static void lu_factor_dd(cdd* A, int* ipiv, int n) {
+    for (int k = 0; k < n; k++) {
+        int kp = k;
+        double amax = cabs2(cdd_round(A[k + k * n]));
+        for (int i = k + 1; i < n; i++) {
+            double absi = cabs2(cdd_round(A[i + k * n]));
+            if (absi > amax) { kp = i; amax = absi; }
+        }
+        ipiv[k] = kp;
+        if (amax != 0) {
+            if (k != kp)
+                for (int j = 0; j < n; j++) { cdd t = A[k + j * n]; A[k + j * n] = A[kp + j * n]; A[kp + j * n] = t; }
+            cdd akk = A[k + k * n];
+            for (int i = k + 1; i < n; i++) A[i + k * n] = cdd_div(A[i + k * n], akk);
+        }
+        for (int j = k + 1; j < n; j++) {
+            cdd akj = A[k + j * n];
+            for (int i = k + 1; i < n; i++) A[i + j * n] = cdd_sub(A[i + j * n], cdd_mul(A[i + k * n], akj));
+        }
+    }
+}
+static void lu_solve_dd(const cdd* A, const int* ipiv, int n, cdd* x) {
+    for (int i = 0; i < n; i++) if (i != ipiv[i]) { cdd t = x[i]; x[i] = x[ipiv[i]]; x[ipiv[i]] = t; }
+    for (int j = 0; j < n; j++) {
+        cdd xj = x[j];
+        for (int i = j + 1; i < n; i++) x[i] = cdd_sub(x[i], cdd_mul(A[i + j * n], xj));
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        cdd xj = cdd_div(x[j], A[j + j * n]);
+        x[j] = xj;
+        for (int i = 0; i < j; i++) x[i] = cdd_sub(x[i], cdd_mul(A[i + j * n], xj));
+    }
+}
+static void dd_inverse(const orc_system* F, int n, const cdd* xd, const double* BP, cball* TB, cplx* Cm) {
+    cball* JB = malloc(sizeof(cball) * n * n);
+    cdd* A = malloc(sizeof(cdd) * n * n);
+    cdd* e = malloc(sizeof(cdd) * n);
+    int* ipiv = malloc(sizeof(int) * n);
+    sys_eval_ball(F->eval_jac, F->fixed_params, xd, BP, n, 1, TB, JB);
+    for (int i = 0; i < n * n; i++) { A[i].re = JB[i].re.m; A[i].im = JB[i].im.m; }
+    lu_factor_dd(A, ipiv, n);
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < n; i++) { e[i].re = DD(i == j ? 1.0 : 0.0, 0.0); e[i].im = DD(0.0, 0.0); }
+        lu_solve_dd(A, ipiv, n, e);
+        for (int i = 0; i < n; i++) Cm[i + j * n] = cdd_round(e[i]);
+    }
+    free(JB); free(A); free(e); free(ipiv);
+}
+
+typedef struct orc_certificate {
+    int32_t certified, real, complex_, newton_code, newton_iters, prec;
+    double contraction, inflation;
+} orc_certificate;
+
+typedef struct { int certified, conj_inside, not_real; double bound, inflation; } kr_out;
+
+/* Δx₀ = C r₀, x₀ = x̃ + [-ε, ε](1 + i), M = C J(x₀) - I, x₁ = (x̃ - Δx₀) - M (x₀ - x̃)
+ * (ε_inflation_krawczyk, src/certification.jl:1225-1287); writes x₁ and x₀.  x̃ is a
+ * double-double vector (low parts zero in the FP64 test, ext = 0); the extended test
+ * encloses x̃ and x₀ - x̃ through balls and floors ε at 512 eps (1 + |x̃|). */
+static kr_out krawczyk(const orc_system* F, int n, const cdd* xd, const cplx* Cm, const icx* R0, const double* IP,
+                       int ext, icx* TI, double* box1, double* box0) {
+    icx *D = malloc(sizeof(icx) * n), *X0 = malloc(sizeof(icx) * n), *dX = malloc(sizeof(icx) * n);
+    icx *XT = malloc(sizeof(icx) * n), *JI = malloc(sizeof(icx) * n * n);
+    kr_out o = {0, 1, 0, 0.0, 0.0};
+    for (int i = 0; i < n; i++) {
+        icx dd_ = ic_zero_();
+        for (int k = 0; k < n; k++) dd_ = icx_muladd_pt(Cm[i + k * n], R0[k], dd_);
+        D[i] = dd_;
+        double hr = xd[i].re.hi, hi_ = xd[i].im.hi;
+        if (!ext) {
+            double eps_i = 512.0 * jmax(icx_mag(dd_), 2.220446049250313e-16);
+            o.inflation = jmax(o.inflation, eps_i);
+            X0[i] = ICX(IV(hr - eps_i, hr + eps_i), IV(hi_ - eps_i, hi_ + eps_i));
+            dX[i] = ICX(i_sub_r(X0[i].re, hr), i_sub_r(X0[i].im, hi_));
+            XT[i] = ICX(IV(hr, hr), IV(hi_, hi_));
+        } else {
+            double eps_i = 512.0 * jmax(icx_mag(dd_), 2.220446049250313e-16 * (1.0 + jmax(fabs(hr), fabs(hi_))));
+            o.inflation = jmax(o.inflation, eps_i);
+            X0[i] = ICX(IV(r_dn(r_dn(hr) - eps_i), r_up(r_up(hr) + eps_i)), IV(r_dn(r_dn(hi_) - eps_i), r_up(r_up(hi_) + eps_i)));
+            dball xr = DB(xd[i].re, 0), xi = DB(xd[i].im, 0);
+            dX[i] = ICX(IV(db_ival(db_sub(DB(DD(X0[i].re.lo, 0), 0), xr)).lo, db_ival(db_sub(DB(DD(X0[i].re.hi, 0), 0), xr)).hi),
+                        IV(db_ival(db_sub(DB(DD(X0[i].im.lo, 0), 0), xi)).lo, db_ival(db_sub(DB(DD(X0[i].im.hi, 0), 0), xi)).hi));
+            XT[i] = ICX(db_ival(xr), db_ival(xi));
+        }
+    }
+    sys_eval_ival(F->eval_jac, F->fixed_params, X0, IP, n, 1, TI, JI);
+    double bound = 0.0;
+    int inside = 1;
+    for (int i = 0; i < n; i++) {
+        ival b = IV(0, 0);
+        icx x1 = ic_zero_();
+        for (int j = 0; j < n; j++) {
+            icx m = ic_zero_();
+            for (int k = 0; k < n; k++) m = icx_muladd_pt(Cm[i + k * n], JI[k + j * n], m);
+            if (i == j) m.re = i_sub_r(m.re, 1.0);
+            b = i_add_r(b, icx_mag(m));
+            x1 = icx_muladd(m, dX[j], x1);
+        }
+        bound = jmax(bound, i_mag(b));
+        x1 = icx_sub(icx_sub(XT[i], D[i]), x1);
+        inside = inside && i_interior(x1.re, X0[i].re) && i_interior(x1.im, X0[i].im);
+        o.conj_inside = o.conj_inside && i_interior(x1.re, X0[i].re) && i_interior(i_neg(x1.im), X0[i].im);
+        o.not_real = o.not_real || !i_in(0.0, x1.im);
+        double* o1 = box1 + 4 * i;
+        o1[0] = x1.re.lo; o1[1] = x1.re.hi; o1[2] = x1.im.lo; o1[3] = x1.im.hi;
+        double* o0 = box0 + 4 * i;
+        o0[0] = X0[i].re.lo; o0[1] = X0[i].re.hi; o0[2] = X0[i].im.lo; o0[3] = X0[i].im.hi;
+    }
+    o.bound = 1.41422 * bound;
+    o.certified = o.bound < 0.7071 && inside;
+    free(D); free(X0); free(dX); free(XT); free(JI);
+    return o;
+}
+
+/* x̃ ← x̃ - C mid(F(x̃)) in double-double with ball residuals (at most 10 steps, until
+ * ‖C F(x̃)‖ stops decreasing; src/certification.jl:1303-1320); R0 encloses F at the final x̃. */
+static void refine_dd(const orc_system* F, int n, cdd* xd, const cplx* Cm, const double* BP, cball* TB, icx* R0) {
+    cball* rb = malloc(sizeof(cball) * n);
+    cplx* dl = cvec(n);
+    double acc = INFINITY;
+    for (int it = 1; it <= 10; it++) {
+        sys_eval_ball(F->eval, F->fixed_params, xd, BP, n, 0, TB, rb);
+        for (int i = 0; i < n; i++) R0[i] = ICX(db_ival(rb[i].re), db_ival(rb[i].im));
+        double nrm = 0.0;
+        for (int i = 0; i < n; i++) {
+            icx d = ic_zero_();
+            for (int k = 0; k < n; k++) d = icx_muladd_pt(Cm[i + k * n], R0[k], d);
+            nrm = jmax(nrm, icx_mag(d));
+            dl[i] = C(0.5 * d.re.lo + 0.5 * d.re.hi, 0.5 * d.im.lo + 0.5 * d.im.hi);
+        }
+        if (!(nrm <= acc) || it == 10) break;
+        acc = nrm;
+        for (int i = 0; i < n; i++) {
+            xd[i].re = dd_add_acc(xd[i].re, DD(-dl[i].re, 0));
+            xd[i].im = dd_add_acc(xd[i].im, DD(-dl[i].im, 0));
+        }
+    }
+    free(rb); free(dl);
+}
+
+static double cert_inf_norm(const cplx* x, int n) {
+    double m = cabs2(x[0]);
+    for (int i = 1; i < n; i++) m = max_fast(m, cabs2(x[i]));
+    return sqrt(m);
+}
+
+/* One candidate: newton(F, x, p, InfNorm(); rtol = 8 eps, atol = 0, extended_precision =
+ * true, max_iters = 8) (src/newton.jl:107-175), C = inverse of the last Jacobian, then the
+ * ε-inflation Krawczyk test. */
+static void certify_one(const orc_system* F, int n, const cplx* x0, const cplx* p, const double* IP, const double* BP,
+                        int refine, int extended, int real_system, orc_certificate* out, double* box1, double* box0,
+                        double* refined) {
+    const orc_tape* ej = F->eval_jac;
+    const orc_tape* ev = F->eval;
+    int space = ej->tape_space > ev->tape_space ? ej->tape_space : ev->tape_space;
+    cplx* T = malloc(sizeof(cplx) * (space + 1));
+    cdd* Td = malloc(sizeof(cdd) * (space + 1));
+    icx* TI = malloc(sizeof(icx) * (space + 1));
+    cplx *x = cvec(n), *r = cvec(n), *LU = cvec(n * n), *Cm = cvec(n * n), *e = cvec(n);
+    cdd *xe = malloc(sizeof(cdd) * n), *re = malloc(sizeof(cdd) * n);
+    icx *X = malloc(sizeof(icx) * n), *R0 = malloc(sizeof(icx) * n);
+    int* ipiv = malloc(sizeof(int) * n);
+    memcpy(x, x0, sizeof(cplx) * n);
+    int code = -1, iters = 0;
+    if (refine) {
+        double norm_x = cert_inf_norm(x, n), tol = 8.0 * 2.220446049250313e-16 * norm_x;
+        double prev = NAN, ac = 1.0;
+        code = 2;
+        iters = 8;
+        for (int it = 1; it <= 8; it++) {
+            sys_eval(ej, x, p, T, r, LU, n);
+            for (int i = 0; i < n; i++) xe[i] = cdd_from(x[i]);
+            sys_eval_dd(ev, xe, p, Td, re);
+            for (int i = 0; i < n; i++) r[i] = C(re[i].re.hi, re[i].im.hi);
+            lu_factor(LU, ipiv, n);
+            lu_solve(LU, ipiv, n, r);
+            double nrm = cert_inf_norm(r, n);
+            for (int i = 0; i < n; i++) x[i] = csub(x[i], r[i]);
+            if (it > 1) {
+                if (nrm < ac * prev) { prev = nrm; ac *= ac; }
+                else { code = nrm < jmax(0.0, tol) ? 0 : 1; iters = it; break; }
+            } else {
+                prev = nrm;
+            }
+        }
+    } else {
+        sys_eval(ej, x, p, T, r, LU, n);
+        lu_factor(LU, ipiv, n);
+    }
+    for (int j = 0; j < n; j++) {
+        for (int i = 0; i < n; i++) e[i] = C(i == j ? 1.0 : 0.0, 0.0);
+        lu_solve(LU, ipiv, n, e);
+        for (int i = 0; i < n; i++) Cm[i + j * n] = e[i];
+    }
+    cdd* xd = malloc(sizeof(cdd) * n);
+    for (int i = 0; i < n; i++) { X[i] = ic_point(x[i]); xd[i] = cdd_from(x[i]); }
+    sys_eval_ival(ev, F->fixed_params, X, IP, n, 0, TI, R0);
+    kr_out o = krawczyk(F, n, xd, Cm, R0, IP, 0, TI, box1, box0);
+    int prec = 53;
+    if (!o.certified && extended) {
+        /* x̃ becomes a double-double point refined with ball residuals, then the test runs
+         * again; if it still fails, C is recomputed from a double-double Jacobian and it runs
+         * once more (extended_prec_certify_solution / arb_ε_inflation_krawczyk,
+         * src/certification.jl:1116-1218, 1289-1394, in double-double). */
+        int tspace = ej->tape_space > space ? ej->tape_space : space;
+        cball* TB = malloc(sizeof(cball) * (tspace + 1));
+        refine_dd(F, n, xd, Cm, BP, TB, R0);
+        o = krawczyk(F, n, xd, Cm, R0, IP, 1, TI, box1, box0);
+        prec = 106;
+        if (!o.certified) {
+            dd_inverse(F, n, xd, BP, TB, Cm);
+            refine_dd(F, n, xd, Cm, BP, TB, R0);
+            o = krawczyk(F, n, xd, Cm, R0, IP, 1, TI, box1, box0);
+        }
+        free(TB);
+    }
+    for (int i = 0; i < n; i++) { refined[2 * i] = xd[i].re.hi; refined[2 * i + 1] = xd[i].im.hi; }
+    free(xd);
+    int is_complex = o.certified && o.not_real;
+    out->certified = o.certified;
+    out->complex_ = is_complex;
+    out->real = o.certified && o.conj_inside && real_system && !is_complex;
+    out->newton_code = code;
+    out->newton_iters = iters;
+    out->prec = prec;
+    out->contraction = o.bound;
+    out->inflation = o.inflation;
+    free(T); free(Td); free(TI); free(x); free(r); free(LU); free(Cm); free(e); free(xe); free(re);
+    free(X); free(R0); free(ipiv);
+}
+
+typedef struct {
+    const orc_system* F;
+    int n, nsol, refine, extended, real_system, npp, np;
+    const double *cand, *IP, *BP;
+    const int32_t* group_of;
+    const cplx* p;
+    orc_certificate* out;
+    double *box1, *box0, *refined;
+    atomic_long next;
+} cert_job_t;
+
+static void* cert_worker(void* arg) {
+    cert_job_t* J = arg;
+    int n = J->n;
+    cplx* x = cvec(n);
+    for (;;) {
+        long k = atomic_fetch_add(&J->next, 1);
+        if (k >= J->nsol) break;
+        for (int i = 0; i < n; i++) x[i] = C(J->cand[2 * (k * n + i)], J->cand[2 * (k * n + i) + 1]);
+        int g = J->group_of ? J->group_of[k] : 0;
+        certify_one(J->F, n, x, J->p + (size_t)g * J->npp, J->IP + (size_t)g * 4 * J->np, J->BP + (size_t)g * 6 * J->np,
+                    J->refine, J->extended, J->real_system, &J->out[k], J->box1 + 4 * k * n, J->box0 + 4 * k * n,
+                    J->refined + 2 * k * n);
+    }
+    free(x);
+    return NULL;
+}
+
+/* Certify nsol candidates (the loop of _certify, src/certification.jl:779-975).  Parameter
+ * groups as in hcg_certify: ngroups points (params), boxes (params_ival, NULL = the points)
+ * and double-double balls (params_ball, NULL = the points); candidate k uses group
+ * group_of[k] (NULL: group 0). */
+int orc_certify(const orc_system* F, int n, int nsol, const double* cand, int ngroups, const double* params,
+                const double* params_ival, const double* params_ball, const int32_t* group_of, int refine,
+                int extended, int real_system, int nthreads, orc_certificate* out, double* box1, double* box0,
+                double* refined) {
+    if (!F || !F->eval || !F->eval_jac || n < 1) { snprintf(orc_err, sizeof orc_err, "bad system"); return -1; }
+    if (orc_tape_check(F->eval) || orc_tape_check(F->eval_jac)) return -1;
+    int np = F->fixed_params ? 0 : F->eval->nparams;
+    if (np > 0 && (!params || ngroups < 1)) { snprintf(orc_err, sizeof orc_err, "params missing"); return -1; }
+    if (np == 0) { ngroups = 1; group_of = NULL; }
+    if (group_of)
+        for (int k = 0; k < nsol; k++)
+            if (group_of[k] < 0 || group_of[k] >= ngroups) { snprintf(orc_err, sizeof orc_err, "bad group"); return -1; }
+    const double* fixed = F->fixed_params;
+    int npp = F->eval->nparams;
+    cplx* p = cvec(npp * ngroups + 1);
+    double* IP = calloc(4 * (size_t)(np * ngroups + 1), sizeof(double));
+    double* BP = calloc(6 * (size_t)(np * ngroups + 1), sizeof(double));
+    for (int g = 0; g < ngroups; g++)
+        for (int k = 0; k < npp; k++)
+            p[g * npp + k] = fixed ? C(fixed[2 * k], fixed[2 * k + 1])
+                                   : C(params[2 * (g * npp + k)], params[2 * (g * npp + k) + 1]);
+    for (long q = 0; q < (long)np * ngroups; q++) {
+        if (params_ival) memcpy(IP + 4 * q, params_ival + 4 * q, 4 * sizeof(double));
+        else { IP[4 * q] = IP[4 * q + 1] = params[2 * q]; IP[4 * q + 2] = IP[4 * q + 3] = params[2 * q + 1]; }
+        if (params_ball) memcpy(BP + 6 * q, params_ball + 6 * q, 6 * sizeof(double));
+        else { BP[6 * q] = params[2 * q]; BP[6 * q + 3] = params[2 * q + 1]; }
+    }
+    if (nthreads < 1) nthreads = 1;
+    cert_job_t J = {F, n, nsol, refine, extended, real_system, npp, np, cand, IP, BP, group_of, p, out, box1, box0,
+                    refined, 0};
+    pthread_t* th = malloc(sizeof(pthread_t) * nthreads);
+    for (int i = 0; i < nthreads; i++) pthread_create(&th[i], NULL, cert_worker, &J);
+    for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+    free(th); free(p); free(IP); free(BP);
+    return 0;
+}
+
+/* One interval operation on (a, b) for the enclosure tests: 0 add, 1 sub, 2 mul, 3 div,
+ * 4 sqr(a), 5 inv(a), 6 muladd(a, b, c), 7 mul by the real a.lo, 8 muladd(a.lo, b, c). */
+void orc_ival_op(int op, const double* a, const double* b, const double* c, double* out) {
+    ival A = IV(a[0], a[1]), B = IV(b[0], b[1]), Cc = IV(c[0], c[1]), R;
+    switch (op) {
+    case 0: R = i_add(A, B); break;
+    case 1: R = i_sub(A, B); break;
+    case 2: R = i_mul(A, B); break;
+    case 3: R = i_div(A, B); break;
+    case 4: R = i_sqr(A); break;
+    case 5: R = i_inv(A); break;
+    case 6: R = i_muladd(A, B, Cc); break;
+    case 7: R = i_mul_r(a[0], B); break;
+    case 8: R = i_muladd_r(a[0], B, Cc); break;
+    default: R = IV(NAN, NAN);
+    }
+    out[0] = R.lo;
+    out[1] = R.hi;
+}
+
+/* One ball operation for the enclosure tests: 0 add, 1 sub, 2 mul; balls as (hi, lo, radius). */
+void orc_ball_op(int op, const double* a, const double* b, double* out) {
+    dball A = DB(DD(a[0], a[1]), a[2]), B = DB(DD(b[0], b[1]), b[2]), R;
+    R = op == 0 ? db_add(A, B) : (op == 1 ? db_sub(A, B) : db_mul(A, B));
+    out[0] = R.m.hi;
+    out[1] = R.m.lo;
+    out[2] = R.r;
 }

@@ -6,7 +6,8 @@ parameter-homotopy test cases, transcribed from its test files:
   conics    test/certification_test.jl:256-289: the 30 rational coefficients of five real
             conics (exact fractions), and conics_3264_sample.txt: the first 4 of the 3,264
             real tangent conics of test/data/3264_real_sols.txt, in the same file format with
-            the count line set to 4.
+            the count line set to 4; conics_3264_real.npy: all 3,264 of them (3264 x 15
+            complex128, the file's values), certified in test/certification_test.jl:290-295.
 
 The vectors are data (inputs and expected outputs); the systems themselves are rebuilt in
 tests/systems.py.  Run from the repository root where the reference checkout exists:
@@ -57,6 +58,12 @@ def main(ref: Path):
     lines = (ref / "test" / "data" / "3264_real_sols.txt").read_text().splitlines()
     nsol = 4
     (HERE / "conics_3264_sample.txt").write_text("\n".join([str(nsol)] + lines[1:1 + 16 * nsol]) + "\n")
+    sys.path.insert(0, str(HERE.parents[1] / "homotopycontinuation.jl_amd"))
+    from hcgpu.io import read_solutions
+    import numpy as np
+    allsols = read_solutions(ref / "test" / "data" / "3264_real_sols.txt")
+    assert allsols.shape == (3264, 15)
+    np.save(HERE / "conics_3264_real.npy", allsols)
     assert [len(data["steiner"][k]) for k in ("p", "q", "s_p", "s_q")] == [30, 30, 15, 15]
     assert [len(data["four_bar"][k]) for k in ("s", "p", "q")] == [24, 16, 16]
     (HERE / "param_cases.json").write_text(json.dumps(data, indent=1))

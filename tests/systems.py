@@ -168,3 +168,20 @@ def framework_454(seed: int = 454):
     p1 = p0 + 1e-3 * rng.standard_normal(p0.shape)
     b1 = sum(a[2 * j + k] * p1[i - 1, k] for j, i in enumerate(free) for k in range(2))
     return System(eqs, 16, 1), np.array([b0]), np.array([b1]), start
+
+
+def certification_simple1():
+    """Simple 1 of test/certification_test.jl:3-8: 18 isolated solutions, 4 of them real."""
+    (x, y), _ = variables(2, 0)
+    f1 = (x**4 + y**4 - 1) * (x**2 + y**2 - 2) + x**5 * y
+    f2 = x**2 + 2 * x * y**2 - 2 * y**2 - 0.5
+    return System([f1, f2], 2, 0)
+
+
+def critical_points():
+    """Critical points of the distance from u to the sextic curve f = 0
+    (test/certification_test.jl:98-110): [x, y] - u - λ ∇f and f, parameters u."""
+    xs, us = variables(3, 2)
+    x, y, lam = xs
+    f = (x**4 + y**4 - 1) * (x**2 + y**2 - 2) + x**5 * y
+    return System([x - us[0] - lam * f.diff(0), y - us[1] - lam * f.diff(1), f], 3, 2)

@@ -56,6 +56,7 @@ STRUCTS = {
     "hcg_path_result": abi.hcg_path_result,
     "hcg_step_log": abi.hcg_step_log,
     "hcg_build_options": abi.hcg_build_options,
+    "hcg_certificate": abi.hcg_certificate,
 }
 
 
