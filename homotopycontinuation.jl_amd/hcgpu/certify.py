@@ -136,8 +136,15 @@ def _part(x):
         rad = _up_float(abs(f - Fraction(hi) - Fraction(lo)))
         if f.denominator == 1 and Fraction(point) == f:
             return point, point, point, hi, lo, rad
-        q = float(f.numerator) / float(f.denominator)
-        return point, math.nextafter(q, -math.inf), math.nextafter(q, math.inf), hi, lo, rad
+        num, den = float(f.numerator), float(f.denominator)
+        if Fraction(num) == f.numerator and Fraction(den) == f.denominator:
+            q = num / den
+            return point, math.nextafter(q, -math.inf), math.nextafter(q, math.inf), hi, lo, rad
+        # numerator or denominator not a double: the tightest enclosure of the exact value
+        d = Fraction(point)
+        blo = point if d <= f else math.nextafter(point, -math.inf)
+        bhi = point if d >= f else math.nextafter(point, math.inf)
+        return point, blo, bhi, hi, lo, rad
     x = float(x)
     return x, x, x, x, 0.0, 0.0
 

@@ -172,6 +172,9 @@ def test_rational_parameters_are_enclosed():
         hi, lo, rad = ball[k, :3]
         assert abs(v - _fr(hi) - _fr(lo)) <= _fr(rad) and rad < 1e-32 and ball[k, 5] == 0.0
     assert tuple(ball[2]) == (0.5, 0.0, 0.0, 0.0, 0.0, 0.0)
+    big = Fraction(3**40 + 1, 7**25)  # numerator and denominator beyond 2^53
+    _, bbox, _ = parameter_boxes([big])
+    assert _fr(bbox[0, 0]) <= big <= _fr(bbox[0, 1]) and bbox[0, 1] <= math.nextafter(bbox[0, 0], math.inf)
 
 
 # ------------------------------------------------------------------ known answers
@@ -314,6 +317,19 @@ def test_parameter_groups_match_separate_calls():
         assert [c.certified for c in rows] == [c.certified for c in one.certificates]
         assert all(np.array_equal(a.I, b.I) for a, b in zip(rows, one.certificates))
     assert r.ncertified() == len(S) and r.nreal_certified() == sum(c.real for c in r.certificates)
+
+
+def test_certify_monodromy_result():
+    """certify(F, monodromy result): the parameters come from the result, as in
+    certify(F, ::MonodromyResult) (src/certification.jl:1470-1477)."""
+    from hcgpu.configs import sparse_param_system
+    from hcgpu.monodromy import monodromy_solve
+    from test_monodromy import oracle_tracker
+    F = sparse_param_system(3, 6, 4, seed=7)
+    m = monodromy_solve(F, seed=11, tracker=oracle_tracker, target_solutions_count=8)
+    from hcgpu.certify import certify
+    r = certify(F, m, certifier=OracleCertifier(F))
+    assert r.ncertified() == 8 and r.ndistinct_certified() == 8
 
 
 def test_positive():

@@ -108,3 +108,13 @@ def test_certify_3264_real_conics(gpu_ctx):
     assert r.ndistinct_real_certified() == 3264 and r.ncertified() == 3264
     r2 = certify(F, np.vstack([S, S[:100]]), conics, ctx=gpu_ctx)
     assert r2.ndistinct_certified() == 3264 and len(r2.duplicates) == 100
+
+
+def test_certify_solve_result(gpu_ctx):
+    """The reference's flow: result = solve(F); certify(F, result) (test/certification_test.jl:3-24)."""
+    from hcgpu.solve import solve
+    F = simple1()
+    res = solve(F, seed=5)
+    assert res.nnonsingular() == 18
+    r = certify(F, res, ctx=gpu_ctx)
+    assert (r.ncertified(), r.nreal_certified(), r.ndistinct_certified()) == (18, 4, 18)
