@@ -294,7 +294,137 @@ def assemble(S: np.ndarray, out, check_distinct: bool = True, kernel_ms: float =
     return CertificationResult(certs, dups, kernel_ms)
 
 
-def distinct_certified_solutions(F: System, S, p: Optional[Sequence] = None, **kw) -> List[np.ndarray]:
-    """The midpoints of the distinct certified boxes among the candidates S
-    (distinct_certified_solutions, src/certification.jl:1752-1766)."""
-    return certify(F, S, p, **kw).distinct_solutions()
+class DistinctCertifiedSolutions:
+    """A growing set of distinct certified solutions of F(x; p) (DistinctCertifiedSolutions,
+    src/certification.jl:1561-1750).  add_solution certifies one candidate; add_solutions
+    certifies a batch in one launch and then inserts the certificates in order, which gives
+    each candidate the status sequential add_solution calls would give it."""
+
+    def __init__(self, F: System, params: Optional[Sequence] = None, *, ctx=None, certifier=None):
+        if params is None and F.nparams:
+            raise ValueError("The given system expects parameters but none are given.")
+        self.F = F
+        self.params = None if params is None else list(params)
+        self._pts, self._boxes, self._balls = parameter_boxes(self.params or [])
+        self._real = is_real_system(F, self._pts)
+        if certifier is None:
+            from .gpu import default_context
+            certifier = (ctx or default_context()).certifier(F)
+        self._certifier = certifier
+        self._certs: List[SolutionCertificate] = []
+        self._box = np.zeros((0, F.nvars, 4))
+        self.processed = self.duplicates = self.not_certified = 0
+
+    def __len__(self):
+        return len(self._certs)
+
+    def ncertified_distinct(self) -> int:
+        return len(self._certs)
+
+    def nprocessed(self) -> int:
+        return self.processed
+
+    def nduplicates(self) -> int:
+        return self.duplicates
+
+    def nnotcertified(self) -> int:
+        return self.not_certified
+
+    def stats(self):
+        return dict(processed=self.processed, certified_distinct=len(self._certs), duplicates=self.duplicates,
+                    not_certified=self.not_certified)
+
+    def certificates(self) -> List[SolutionCertificate]:
+        return list(self._certs)
+
+    def solutions(self) -> List[np.ndarray]:
+        return [c.solution for c in self._certs]
+
+    def empty(self):
+        self._certs, self._box = [], np.zeros((0, self.F.nvars, 4))
+        self.processed = self.duplicates = self.not_certified = 0
+        return self
+
+    def _containing(self, x: np.ndarray) -> Optional[SolutionCertificate]:
+        """A distinct certificate whose box contains the point x (guaranteed_duplicate_certificate)."""
+        if not len(self._certs):
+            return None
+        B = self._box
+        inside = ((B[:, :, 0] <= x.real) & (x.real <= B[:, :, 1]) & (B[:, :, 2] <= x.imag) & (x.imag <= B[:, :, 3])).all(axis=1)
+        k = np.flatnonzero(inside)
+        return self._certs[int(k[0])] if len(k) else None
+
+    def _overlapping(self, b: np.ndarray) -> Optional[SolutionCertificate]:
+        if not len(self._certs):
+            return None
+        B = self._box
+        ov = ((B[:, :, 0] <= b[:, 1]) & (b[:, 0] <= B[:, :, 1]) & (B[:, :, 2] <= b[:, 3]) & (b[:, 2] <= B[:, :, 3])).all(axis=1)
+        k = np.flatnonzero(ov)
+        return self._certs[int(k[0])] if len(k) else None
+
+    def _insert(self, c: SolutionCertificate):
+        """add_certificate! (src/certification.jl:351-375): (added, representative)."""
+        other = self._overlapping(c.I)
+        if other is not None:
+            return False, other
+        self._certs.append(c)
+        self._box = np.concatenate([self._box, c.I[None]], axis=0)
+        return True, c
+
+    def add_solutions(self, S, indices: Optional[Sequence[int]] = None):
+        """Add candidates S (distinct_certified_solutions!, src/certification.jl:1768-1860);
+        returns (added, status, representative index, certified solution) per candidate,
+        status one of "certified_distinct", "duplicate", "not_certified"."""
+        S = np.asarray(S, dtype=np.complex128).reshape(-1, self.F.nvars)
+        idx = list(range(1, len(S) + 1)) if indices is None else [int(i) for i in indices]
+        out: List = [None] * len(S)
+        todo = []
+        for k, x in enumerate(S):
+            c = self._containing(x)
+            if c is not None:
+                self.processed += 1
+                self.duplicates += 1
+                out[k] = (False, "duplicate", c.index, None)
+            else:
+                todo.append(k)
+        if todo:
+            has = self.F.nparams > 0
+            o = self._certifier.certify(S[todo], self._pts if has else None, self._boxes if has else None,
+                                        self._balls if has else None, real_system=self._real)
+            r = assemble(S[todo], o, check_distinct=False)
+            for j, k in enumerate(todo):
+                c = r.certificates[j]
+                c.index = idx[k]
+                self.processed += 1
+                if not c.certified:
+                    self.not_certified += 1
+                    out[k] = (False, "not_certified", 0, None)
+                    continue
+                added, rep = self._insert(c)
+                if added:
+                    out[k] = (True, "certified_distinct", c.index, c.solution)
+                else:
+                    self.duplicates += 1
+                    out[k] = (False, "duplicate", rep.index, None)
+        return out
+
+    def add_solution(self, sol, i: int = 0):
+        """add_solution! (src/certification.jl:1673-1719)."""
+        return self.add_solutions([sol], [i])[0]
+
+    def merge(self, other: "DistinctCertifiedSolutions") -> "DistinctCertifiedSolutions":
+        """merge! (src/certification.jl:1735-1745): other's certificates added to this set."""
+        for c in other.certificates():
+            added, _ = self._insert(c)
+            self.processed += 1
+            if not added:
+                self.duplicates += 1
+        return self
+
+
+def distinct_certified_solutions(F: System, S, p: Optional[Sequence] = None, **kw) -> DistinctCertifiedSolutions:
+    """DistinctCertifiedSolutions of the candidates S (distinct_certified_solutions,
+    src/certification.jl:1752-1766)."""
+    d = DistinctCertifiedSolutions(F, p, **kw)
+    d.add_solutions(S)
+    return d

@@ -332,6 +332,43 @@ def test_certify_monodromy_result():
     assert r.ncertified() == 8 and r.ndistinct_certified() == 8
 
 
+def test_distinct_certified_solutions_incremental_api():
+    """test/certification_test.jl:164-247: statuses, representatives and stats of
+    add_solution!, batches and merge! (the interval-tree collision case needs no special
+    handling here: boxes are compared directly)."""
+    from hcgpu.certify import DistinctCertifiedSolutions, distinct_certified_solutions
+    (x, y), _ = variables(2, 0)
+    F = System([x**2 + y**2 - 1, x - y], 2, 0)
+    sols = solve_nonsingular(F)
+    assert len(sols) == 2
+    d = DistinctCertifiedSolutions(F, None, certifier=OracleCertifier(F))
+    added, status, rep, _ = d.add_solution(sols[0], 1)
+    assert (added, status, rep) == (True, "certified_distinct", 1)
+    added, status, rep, _ = d.add_solution(sols[0], 2)
+    assert (added, status, rep) == (False, "duplicate", 1)
+    C = critical_points_system()
+    nc = DistinctCertifiedSolutions(C, [-0.32, -0.1], certifier=OracleCertifier(C))
+    rng = np.random.default_rng(1)
+    added, status, rep, _ = nc.add_solution(100 * (rng.standard_normal(3) + 1j * rng.standard_normal(3)) / np.sqrt(2), 3)
+    assert (added, status, rep) == (False, "not_certified", 0)
+    assert d.stats() == dict(processed=2, certified_distinct=1, duplicates=1, not_certified=0)
+    assert nc.stats() == dict(processed=1, certified_distinct=0, duplicates=0, not_certified=1)
+    (z,), (u,) = variables(1, 1)
+    Z = System([z**2 - u], 1, 1)
+    col = DistinctCertifiedSolutions(Z, [1.0], certifier=OracleCertifier(Z))
+    assert col.add_solution([1.0 + 0j], 1)[:3] == (True, "certified_distinct", 1)
+    assert col.add_solution([-1.0 + 0j], 2)[:3] == (True, "certified_distinct", 2)
+    assert np.allclose(sorted(float(s[0].real) for s in col.solutions()), [-1.0, 1.0], rtol=0, atol=1e-15)
+    b = DistinctCertifiedSolutions(F, None, certifier=OracleCertifier(F))
+    b.add_solutions(sols[:1])
+    b.add_solutions([sols[1], sols[0]])
+    assert len(b.solutions()) == 2 and b.ncertified_distinct() == 2 and b.nduplicates() == 1
+    left = distinct_certified_solutions(F, sols[:1], certifier=OracleCertifier(F))
+    right = distinct_certified_solutions(F, [sols[1], sols[0]], certifier=OracleCertifier(F))
+    left.merge(right)
+    assert len(left.solutions()) == 2 and left.nduplicates() == 1
+
+
 def test_positive():
     (x, y), _ = variables(2, 0)
     F = System([x**2 + y**2 - 1, x - y], 2, 0)

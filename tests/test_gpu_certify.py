@@ -108,6 +108,14 @@ def test_certify_3264_real_conics(gpu_ctx):
     assert r.ndistinct_real_certified() == 3264 and r.ncertified() == 3264
     r2 = certify(F, np.vstack([S, S[:100]]), conics, ctx=gpu_ctx)
     assert r2.ndistinct_certified() == 3264 and len(r2.duplicates) == 100
+    # DistinctCertifiedSolutions (test/certification_test.jl:300-314): one at a time, and batched
+    from hcgpu.certify import DistinctCertifiedSolutions, distinct_certified_solutions
+    d = DistinctCertifiedSolutions(F, conics, ctx=gpu_ctx)
+    for x in S[:40]:
+        d.add_solution(x, 1)
+    assert len(d.solutions()) == 40
+    d2 = distinct_certified_solutions(F, np.vstack([S, S[:100]]), conics, ctx=gpu_ctx)
+    assert len(d2.solutions()) == 3264 and d2.nduplicates() == 100
 
 
 def test_certify_solve_result(gpu_ctx):
