@@ -163,9 +163,8 @@ def certify_leg(ctx, F, Q, sol, rec, groups, args):
     r = certify_groups(F, S, Q, G, certifier=cert)
     wall = time.perf_counter() - t0
     kms = ctx.last_kernel_ms()
-    prec = np.array([c.prec for c in r.certificates])
     leg = {"candidates": int(len(S)), "certified": r.ncertified(), "distinct_certified": r.ndistinct_certified(),
-           "by_double_double_retry": int(((prec == 106) & np.array([c.certified for c in r.certificates])).sum()),
+           "by_double_double_retry": int(((r.prec == 106) & r.certified).sum()),
            "kernel": "hcg_certify_kernel", "kernel_ms": round(kms, 3),
            "value": round(len(S) / (kms * 1e-3), 1), "unit": "certificates/s",
            "host_call_s": round(wall, 3)}

@@ -64,33 +64,59 @@ class SolutionCertificate:
 
 
 class CertificationResult:
-    """certificates (in candidate order) and duplicates: lists of 1-based candidate indices,
-    the distinct representative first (CertificationResult, src/certification.jl:436-441)."""
+    """Certificates in candidate order and duplicates (lists of 1-based candidate indices, the
+    distinct representative first) (CertificationResult, src/certification.jl:436-441).  The
+    per-candidate arrays of the launch are kept as arrays (`certified`, `real`, `complex`,
+    `prec`, boxes `box1` / `box0`); SolutionCertificate objects are built on first access."""
 
-    def __init__(self, certificates: List[SolutionCertificate], duplicates: List[List[int]], kernel_ms: float = 0.0):
-        self.certificates = certificates
+    def __init__(self, S: np.ndarray, out, duplicates: List[List[int]], kernel_ms: float = 0.0):
+        self.candidates = S
+        self.certified = out.certified.astype(bool)
+        self.real = out.real.astype(bool)
+        self.complex = out.complex_.astype(bool)
+        self.prec = out.prec.astype(np.int32)
+        self.newton_code = out.newton_code
+        self.contraction = out.contraction
+        self.box1, self.box0 = out.box1, out.box0
         self.duplicates = duplicates
         self.kernel_ms = kernel_ms
+        self._certs: Optional[List[SolutionCertificate]] = None
+
+    def _cert(self, k: int) -> SolutionCertificate:
+        ok = bool(self.certified[k])
+        b1 = self.box1[k]
+        mid = ((b1[:, 0] + b1[:, 1]) / 2 + 1j * (b1[:, 2] + b1[:, 3]) / 2) if ok else None
+        return SolutionCertificate(solution_candidate=self.candidates[k], certified=ok, real=bool(self.real[k]),
+                                   complex=bool(self.complex[k]), index=k + 1, I=b1, I0=self.box0[k], solution=mid,
+                                   newton_code=int(self.newton_code[k]), contraction=float(self.contraction[k]),
+                                   prec=int(self.prec[k]))
+
+    @property
+    def certificates(self) -> List[SolutionCertificate]:
+        if self._certs is None:
+            self._certs = [self._cert(k) for k in range(len(self.candidates))]
+        return self._certs
+
+    def _dups(self, flags) -> int:
+        return sum(len(d) - 1 for d in self.duplicates if flags[d[0] - 1])
 
     def ncertified(self) -> int:
-        return sum(c.certified for c in self.certificates)
+        return int(self.certified.sum())
 
     def nreal_certified(self) -> int:
-        return sum(c.certified and c.real for c in self.certificates)
+        return int((self.certified & self.real).sum())
 
     def ncomplex_certified(self) -> int:
-        return sum(c.certified and c.complex for c in self.certificates)
+        return int((self.certified & self.complex).sum())
 
     def ndistinct_certified(self) -> int:
         return self.ncertified() - sum(len(d) - 1 for d in self.duplicates)
 
     def ndistinct_real_certified(self) -> int:
-        return self.nreal_certified() - sum(len(d) - 1 for d in self.duplicates
-                                            if self.certificates[d[0] - 1].real)
+        return self.nreal_certified() - self._dups(self.real)
 
     def ndistinct_complex_certified(self) -> int:
-        return self.ncomplex_certified() - sum(len(d) - 1 for d in self.duplicates
-                                               if self.certificates[d[0] - 1].complex)
+        return self.ncomplex_certified() - self._dups(self.complex)
 
     def distinct_certificates(self) -> List[SolutionCertificate]:
         drop = {k for d in self.duplicates for k in d[1:]}
@@ -101,7 +127,7 @@ class CertificationResult:
 
     def __str__(self):
         return ("CertificationResult\n===================\n"
-                f"• {len(self.certificates)} solution candidates given\n"
+                f"• {len(self.candidates)} solution candidates given\n"
                 f"• {self.ncertified()} certified solution intervals ({self.nreal_certified()} real, "
                 f"{self.ncomplex_certified()} complex)\n"
                 f"• {self.ndistinct_certified()} distinct certified solution intervals "
@@ -281,17 +307,8 @@ def certify_groups(F: System, S, parameter_sets: Sequence[Sequence], group_of, *
 
 def assemble(S: np.ndarray, out, check_distinct: bool = True, kernel_ms: float = 0.0) -> CertificationResult:
     """CertificationResult from the per-candidate arrays of a certification launch."""
-    certs = []
-    for k in range(len(S)):
-        ok = bool(out.certified[k])
-        b1 = out.box1[k]
-        mid = ((b1[:, 0] + b1[:, 1]) / 2 + 1j * (b1[:, 2] + b1[:, 3]) / 2) if ok else None
-        certs.append(SolutionCertificate(solution_candidate=S[k], certified=ok, real=bool(out.real[k]),
-                                         complex=bool(out.complex_[k]), index=k + 1, I=b1, I0=out.box0[k],
-                                         solution=mid, newton_code=int(out.newton_code[k]),
-                                         contraction=float(out.contraction[k]), prec=int(out.prec[k])))
     dups = duplicate_groups(out.certified.astype(bool), out.box1) if check_distinct else []
-    return CertificationResult(certs, dups, kernel_ms)
+    return CertificationResult(S, out, dups, kernel_ms)
 
 
 class DistinctCertifiedSolutions:
