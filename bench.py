@@ -190,6 +190,43 @@ def certify_leg(ctx, F, Q, sol, rec, groups, args):
     return leg
 
 
+def training_leg(ctx, F, p0, S, args):
+    """The learned proposer's training workload: step records logged by the plain tracker on
+    16 training targets (seeded draws disjoint from the benchmark's), then Adam on the GPU
+    (hcg_mlp_train) for 2 epochs; throughput in records per second (outside the timed region)."""
+    import numpy as np
+    from hcgpu import learned
+    from hcgpu.configs import parameter_draws
+    from hcgpu.gpu import track_batch
+    from hcgpu.homotopies import ParameterHomotopy
+    from hcgpu.solve import make_options
+    Qt = parameter_draws(16, 16, seed=30)
+    Ht = ParameterHomotopy(F, p0, Qt, lift_affine=True)
+    st = np.tile(S, (len(Qt), 1))
+    gt = np.repeat(np.arange(len(Qt), dtype=np.int32), len(S))
+    o = track_batch(ctx.program(Ht), st, make_options(), group_of_path=gt, log_capacity=2_000_000)
+    X, Y, W = learned.training_set(o.log, o.code)
+    mu, sd = learned.feature_scaling(X)
+    Xn = ((X - mu) / sd).astype(np.float32)
+    epochs = 2
+    t0 = time.perf_counter()
+    _, loss = learned.train(ctx, learned.init_params(0), Xn, Y, W, epochs=epochs, batch=4096, lr=2e-3, seed=0)
+    wall = time.perf_counter() - t0
+    leg = {"records": int(len(X)), "epochs": epochs, "batch": 4096, "value": round(len(X) * epochs / wall, 1),
+           "unit": "records/s (forward + backward + Adam)", "wall_s": round(wall, 3), "final_loss": round(float(loss), 5)}
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        from oracle import mlp_ref  # CPU baseline only
+        m = 4096
+        t0 = time.perf_counter()
+        mlp_ref.loss_and_grad(learned.init_params(0), Xn[:m].astype(np.float64), Y[:m].astype(np.float64),
+                              W[:m].astype(np.float64))
+        dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "records/s (forward + backward)", "cores": 1,
+                               "kind": "port", "sample": f"one batch of {m} records, numpy float64 reference, {dt:.2f} s"}
+    return leg
+
+
 def bench_param10(args, out):
     import numpy as np
     from hcgpu import learned
@@ -242,6 +279,8 @@ def bench_param10(args, out):
     })
     if not args.no_certify:
         out["certification"] = certify_leg(ctx, F, Q, sol_l, rec_l, groups, args)
+    if not args.no_training:
+        out["learned_training"] = training_leg(ctx, F, p0, S, args)
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
         import oracle  # CPU baseline only (never the measured GPU path)
@@ -364,6 +403,7 @@ def main():
     ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
+    ap.add_argument("--no-training", action="store_true", help="skip the proposer-training leg (param10)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-targets", type=int, default=32, help="targets of the CPU baseline sample (param10)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
