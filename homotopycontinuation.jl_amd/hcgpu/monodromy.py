@@ -5,9 +5,10 @@ Follows the reference's monodromy_solve (src/monodromy.jl): a loop p -> p1 -> p2
 p1, p2 drawn by `independent_normal` (MonodromyLoop, :103-117); every known solution is
 tracked around each new loop (track, :1514-1597); endpoints back at p that are not known
 yet are added; the search stops when `target_solutions_count` solutions are known or
-`max_loops_no_progress` loops in a row found nothing new (:1151-1160).  A start pair for a
-system linear in its parameters comes from a random x and a linear solve for p
-(find_start_pair_linear_in_params, :855-914).
+`max_loops_no_progress` loops in a row found nothing new (:1151-1160).  A start pair comes
+from a random x and a linear solve for p when the system is linear in its parameters
+(find_start_pair_linear_in_params, :855-914), else from Newton's method in (x, p)
+(:827-853).
 
 Each loop edge is ONE batched launch over all known solutions (a parameter homotopy from
 the edge's start to its end parameters), on the GPU by default.  New solutions are
@@ -29,12 +30,11 @@ def independent_normal(rng: np.random.Generator, p: np.ndarray) -> np.ndarray:
     return (rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)) / np.sqrt(2)
 
 
-def find_start_pair(F: System, rng: np.random.Generator) -> Tuple[np.ndarray, np.ndarray]:
-    """(x0, p0) with F(x0; p0) = 0 for a system linear in its parameters: x0 random, p0 a
-    least-squares solution of the linear system F(x0; p) = 0 in p."""
+def _linear_start_pair(F: System, rng: np.random.Generator) -> Optional[Tuple[np.ndarray, np.ndarray]]:
+    """find_start_pair_linear_in_params (src/monodromy.jl:855-880): x0 random, p0 the
+    least-squares solution of the linear system F(x0; p) = 0; None when F is not linear in
+    its parameters or the linear system has more equations than parameters."""
     n, m = F.nvars, F.nparams
-    if m == 0:
-        raise ValueError("the system has no parameters")
     x0 = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
     A = np.zeros((len(F.polys), m), dtype=np.complex128)
     b = np.zeros(len(F.polys), dtype=np.complex128)
@@ -42,18 +42,69 @@ def find_start_pair(F: System, rng: np.random.Generator) -> Tuple[np.ndarray, np
         for e, c in f.terms.items():
             xe, pe = e[:n], e[n:]
             if sum(pe) > 1:
-                raise ValueError("the system is not linear in its parameters")
+                return None
             v = c * np.prod(x0 ** np.asarray(xe))
             if sum(pe) == 0:
                 b[i] -= v
             else:
                 A[i, int(np.argmax(pe))] += v
-    if len(F.polys) > m:
-        raise ValueError("more equations than parameters: no start pair from a linear solve")
-    if not np.all(np.any(A != 0, axis=1)):
-        raise ValueError("an equation does not involve the parameters: no start pair from a linear solve")
-    p0 = np.linalg.lstsq(A, b, rcond=None)[0]
+    if len(F.polys) > m or not np.all(np.any(A != 0, axis=1)):
+        return None  # more equations than parameters, or an equation without parameters
+    if not np.any(b):
+        if len(F.polys) == m:
+            return None  # only p = 0 (the reference does not take 0 as a parameter)
+        _, _, vh = np.linalg.svd(A)
+        N = vh[len(F.polys):].conj().T
+        p0 = N @ ((rng.standard_normal(N.shape[1]) + 1j * rng.standard_normal(N.shape[1])) / np.sqrt(2))
+    else:
+        p0 = np.linalg.lstsq(A, b, rcond=None)[0]
     return x0, p0
+
+
+def _newton_start_pair(F: System, rng: np.random.Generator, max_tries: int = 1000, rtol: float = 1e-12,
+                       max_iters: int = 50) -> Optional[Tuple[np.ndarray, np.ndarray]]:
+    """find_start_pair(F::AbstractSystem) (src/monodromy.jl:827-853): Newton's method in
+    variable and parameter space from random (x, p), with minimum-norm steps for the
+    underdetermined system, restarted from new random points until it converges."""
+    n, m = F.nvars, F.nparams
+    N = n + m
+    J = [[f.diff(k) for k in range(N)] for f in F.polys]
+    deg = max(max(sum(e) for e in f.terms) for f in F.polys)
+    for _ in range(max_tries):
+        z = (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / np.sqrt(2)
+        prev = np.inf
+        for _ in range(max_iters):
+            zl = list(z)
+            r = np.array([f(zl) for f in F.polys], dtype=np.complex128)
+            Jz = np.array([[d(zl) for d in row] for row in J], dtype=np.complex128)
+            dz = np.linalg.lstsq(Jz, r, rcond=None)[0]
+            z = z - dz
+            step = np.abs(dz).max()
+            if step <= rtol * max(1.0, np.abs(z).max()):
+                # a stationary point of |F| that is not a zero (rank-deficient Jacobian) is
+                # rejected, as the reference's second Newton run at fixed p would reject it
+                zl = list(z)
+                res = max(abs(f(zl)) for f in F.polys)
+                if res <= 1e-9 * max(1.0, np.abs(z).max()) ** deg:
+                    return z[:n], z[n:]
+                break
+            if not np.isfinite(step) or (step > prev and step > 1e-6):
+                break
+            prev = step
+    return None
+
+
+def find_start_pair(F: System, rng: np.random.Generator) -> Tuple[np.ndarray, np.ndarray]:
+    """(x0, p0) with F(x0; p0) = 0 (find_start_pair, src/monodromy.jl:814-853): the linear
+    solve when F is linear in its parameters, else Newton's method in (x, p)."""
+    if F.nparams == 0:
+        raise ValueError("the system has no parameters")
+    pair = _linear_start_pair(F, rng)
+    if pair is None:
+        pair = _newton_start_pair(F, rng)
+    if pair is None:
+        raise ValueError("no start pair found")
+    return pair
 
 
 @dataclass

@@ -185,3 +185,25 @@ def critical_points():
     x, y, lam = xs
     f = (x**4 + y**4 - 1) * (x**2 + y**2 - 2) + x**5 * y
     return System([x - us[0] - lam * f.diff(0), y - us[1] - lam * f.diff(1), f], 3, 2)
+
+
+def toric_ed(A):
+    """Critical points of the distance to a toric variety (test/monodromy_test.jl:4-12):
+    unknowns t[1:d], y[1:n], parameters u[1:n]; equations φ + y − u and Dφᵀ y with
+    φ_j = Π_i t_i^{A[i][j]}.  For A = [3 2 1 0; 0 1 2 3] (the twisted cubic) 21 solutions."""
+    d, n = len(A), len(A[0])
+    xs, us = variables(d + n, n)
+    t, y = xs[:d], xs[d:]
+    phi = []
+    for j in range(n):
+        m = t[0] ** A[0][j]
+        for i in range(1, d):
+            m = m * t[i] ** A[i][j]
+        phi.append(m)
+    eqs = [phi[j] + y[j] - us[j] for j in range(n)]
+    for k in range(d):
+        acc = phi[0].diff(k) * y[0]
+        for j in range(1, n):
+            acc = acc + phi[j].diff(k) * y[j]
+        eqs.append(acc)
+    return System(eqs, d + n, n)

@@ -54,3 +54,16 @@ def test_monodromy_recovers_parameter_workload_starts(gpu_ctx):
           f"new per loop {r.new_per_loop}")
     assert r.returncode == "success" and r.nsolutions() == len(S)
     assert len(_new_points(S, r.solutions, 1e-8)) == 0 and len(_new_points(r.solutions, S, 1e-8)) == 0
+
+
+def test_monodromy_toric_ed_gpu(gpu_ctx):
+    """test/monodromy_test.jl:14-31: the twisted cubic's 21 ED critical points, on the GPU,
+    identical to the same run on the oracle's tracker."""
+    import systems
+    F = systems.toric_ed([[3, 2, 1, 0], [0, 1, 2, 3]])
+    g = monodromy_solve(F, seed=5, tracker=gpu_edge_tracker(gpu_ctx), target_solutions_count=21,
+                        max_loops_no_progress=50)
+    o = monodromy_solve(F, seed=5, tracker=_oracle_tracker, target_solutions_count=21, max_loops_no_progress=50)
+    assert g.returncode == "success" and g.nsolutions() == 21
+    assert g.new_per_loop == o.new_per_loop
+    np.testing.assert_array_equal(g.solutions, o.solutions)

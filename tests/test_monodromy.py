@@ -39,15 +39,16 @@ def same_set(A, B, tol=1e-8):
 
 
 def test_find_start_pair_is_a_solution():
-    F = sparse_param_system(3, 6, 4, seed=7)
-    x0, p0 = find_start_pair(F, np.random.default_rng(1))
-    res = max(abs(f(list(x0) + list(p0))) for f in F.polys)
-    assert res < 1e-12
-    with pytest.raises(ValueError):  # Steiner: the tangent-conic equations have no parameters
-        find_start_pair(systems.steiner()[0], np.random.default_rng(2))
+    """Linear solve for systems linear in their parameters; Newton's method in (x, p) for
+    the others (Steiner's tangency equations have no parameters, x² − a² is quadratic in a)."""
     (x,), (a,) = variables(1, 1)
+    for F in [sparse_param_system(3, 6, 4, seed=7), systems.steiner()[0], System([x**2 - a * a], 1, 1),
+              systems.toric_ed([[3, 2, 1, 0], [0, 1, 2, 3]])]:
+        x0, p0 = find_start_pair(F, np.random.default_rng(1))
+        scale = max(1.0, np.abs(np.concatenate([x0, p0])).max())
+        assert max(abs(f(list(x0) + list(p0))) for f in F.polys) < 1e-10 * scale ** 6
     with pytest.raises(ValueError):
-        find_start_pair(System([x**2 - a * a], 1, 1), np.random.default_rng(1))
+        find_start_pair(System([x**2 - 1], 1, 0), np.random.default_rng(1))
 
 
 def test_monodromy_closed_form():
@@ -75,3 +76,20 @@ def test_monodromy_equals_total_degree_solve():
     assert same_set(r.solutions, o.solution[ok], 1e-8)
     for x in r.solutions:
         assert max(abs(f(list(x) + list(r.parameters))) for f in F.polys) < 1e-10
+
+
+def test_monodromy_toric_ed_21():
+    """test/monodromy_test.jl:14-31, 347-356: the twisted cubic's ED degree is 21; monodromy
+    from one start pair (found by Newton's method in (x, p)) finds all 21, and tracking them
+    to new parameters keeps 21 solutions."""
+    F = systems.toric_ed([[3, 2, 1, 0], [0, 1, 2, 3]])
+    r = monodromy_solve(F, seed=5, tracker=oracle_tracker, target_solutions_count=21, max_loops_no_progress=50)
+    assert r.returncode == "success" and r.nsolutions() == 21
+    assert len(_new_points(np.zeros((0, 6), dtype=complex), r.solutions, 1e-8)) == 21  # pairwise distinct
+    for x in r.solutions:
+        assert max(abs(f(list(x) + list(r.parameters))) for f in F.polys) < 1e-9
+    q = (np.random.default_rng(9).standard_normal(4) + 1j * np.random.default_rng(10).standard_normal(4)) / np.sqrt(2)
+    from hcgpu.homotopies import ParameterHomotopy
+    H = ParameterHomotopy(F, r.parameters, q)
+    o = oracle.track_batch(H, r.solutions, make_options())
+    assert ((o.code == 1) & (~o.singular)).sum() == 21
