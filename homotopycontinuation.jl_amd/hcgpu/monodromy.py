@@ -160,8 +160,12 @@ def gpu_edge_tracker(ctx=None, options=None) -> EdgeTracker:
 def monodromy_solve(F: System, start_solutions: Optional[Sequence[Sequence[complex]]] = None,
                     start_parameters: Optional[Sequence[complex]] = None, *, target_solutions_count: Optional[int] = None,
                     max_loops_no_progress: int = 5, max_loops: int = 100, seed: int = 0, lift_affine: bool = False,
-                    dedup_rtol: float = 1e-8, tracker: Optional[EdgeTracker] = None) -> MonodromyResult:
-    """monodromy_solve(F, [x0], p0; target_solutions_count, max_loops_no_progress, seed)."""
+                    dedup_rtol: float = 1e-8, tracker: Optional[EdgeTracker] = None,
+                    parameter_sampler: Optional[Callable[[np.random.Generator, np.ndarray], np.ndarray]] = None
+                    ) -> MonodromyResult:
+    """monodromy_solve(F, [x0], p0; target_solutions_count, max_loops_no_progress, seed,
+    parameter_sampler): parameter_sampler(rng, p0) draws the loop's intermediate parameters
+    (default independent_normal)."""
     rng = np.random.default_rng(seed)
     if start_solutions is None:
         x0, p0 = find_start_pair(F, rng)
@@ -178,7 +182,8 @@ def monodromy_solve(F: System, start_solutions: Optional[Sequence[Sequence[compl
     for loop in range(max_loops):
         if target_solutions_count is not None and len(known) >= target_solutions_count:
             break
-        p1, p2 = independent_normal(rng, p0), independent_normal(rng, p0)
+        sample = parameter_sampler or independent_normal
+        p1, p2 = np.asarray(sample(rng, p0), dtype=np.complex128), np.asarray(sample(rng, p0), dtype=np.complex128)
         X = known
         for pa, pb in ((p0, p1), (p1, p2), (p2, p0)):
             if not len(X):
@@ -197,3 +202,74 @@ def monodromy_solve(F: System, start_solutions: Optional[Sequence[Sequence[compl
     if target_solutions_count is not None and len(known) >= target_solutions_count:
         res.returncode = "success"
     return res
+
+
+def _substituted_system(F: System) -> System:
+    """[F(x, p + λ v); (a·x − 1) λ + t] in variables [x; λ] and parameters [t; p; v; a]
+    (the verify_system of verify_solution_completeness, src/monodromy.jl:1711-1715)."""
+    from .polys import variables
+    n, m = F.nvars, F.nparams
+    xs, ps = variables(n + 1, 1 + 2 * m + n)
+    x, lam = xs[:n], xs[n]
+    t, p, v, a = ps[0], ps[1:1 + m], ps[1 + m:1 + 2 * m], ps[1 + 2 * m:]
+    subs = list(x) + [p[k] + lam * v[k] for k in range(m)]
+    eqs = []
+    for f in F.polys:
+        acc = None
+        for e, c in f.terms.items():
+            term = None
+            for sym, k in zip(subs, e):
+                if k:
+                    fac = sym ** k
+                    term = fac if term is None else term * fac
+            term = c if term is None else term * c
+            acc = term if acc is None else acc + term
+        eqs.append(acc)
+    lin = a[0] * x[0]
+    for i in range(1, n):
+        lin = lin + a[i] * x[i]
+    eqs.append((lin - 1) * lam + t)
+    return System(eqs, n + 1, 1 + 2 * m + n)
+
+
+def verify_solution_completeness(F: System, sols, q, *, trace_tol: float = 1e-14, seed: int = 0,
+                                 tracker: Optional[EdgeTracker] = None, monodromy_options=None) -> Optional[bool]:
+    """The trace test of verify_solution_completeness (src/monodromy.jl:1694-1806): True if
+    `sols` are all the isolated solutions of F(x; q), False if the trace test fails, None if
+    a solution was lost on the way.  Every homotopy runs as a batched launch (tracker)."""
+    rng = np.random.default_rng(seed)
+    n, m = F.nvars, F.nparams
+    sols = np.asarray(sols, dtype=np.complex128).reshape(-1, n)
+    q = np.asarray(q, dtype=np.complex128)
+    tracker = tracker or gpu_edge_tracker()
+    V = _substituted_system(F)
+    # start witnesses: n solutions moved to random parameters qq, on the hyperplane a0·x = 1
+    qq = (rng.standard_normal(m) + 1j * rng.standard_normal(m)) / np.sqrt(2)
+    HF = ParameterHomotopy(F, q, [q])
+    codes, X = tracker(HF, sols[:min(n, len(sols))], q, qq)
+    Y = X[np.asarray(codes) == 1]
+    a0 = np.linalg.lstsq(Y, np.ones(len(Y), dtype=np.complex128), rcond=None)[0]
+    base = np.concatenate([q, qq - q, a0])
+    start = np.concatenate([[0.0], base]).astype(np.complex128)
+    Y1 = np.hstack([Y, np.ones((len(Y), 1), dtype=np.complex128)])
+    # more witnesses on the λ ≠ 0 component by monodromy with t = 0
+    sampler = lambda r, p: np.concatenate([[0.0], (r.standard_normal(len(p) - 1) +  # noqa: E731
+                                                  1j * r.standard_normal(len(p) - 1)) / np.sqrt(2)])
+    mres = monodromy_solve(V, Y1, start, seed=seed + 1, tracker=tracker, parameter_sampler=sampler,
+                           **(monodromy_options or {}))
+    S = np.vstack([np.hstack([sols, np.zeros((len(sols), 1), dtype=np.complex128)]), mres.solutions])
+    gamma = complex(rng.standard_normal(), rng.standard_normal()) / np.sqrt(2)
+    HV = ParameterHomotopy(V, start, [start])
+    mid = start.copy()
+    mid[0] = 0.5 * gamma
+    end = start.copy()
+    end[0] = gamma
+    c1, S1 = tracker(HV, S, start, mid)
+    if (np.asarray(c1) == 1).sum() != len(S):
+        return None
+    c2, S2 = tracker(HV, S1, mid, end)
+    if (np.asarray(c2) == 1).sum() != len(S):
+        return None
+    M = np.vstack([np.stack([S.sum(axis=0), S1.sum(axis=0), S2.sum(axis=0)], axis=1), np.ones((1, 3))])
+    sv = np.linalg.svd(M, compute_uv=False)
+    return bool(sv[2] / sv[0] < trace_tol)

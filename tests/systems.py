@@ -207,3 +207,9 @@ def toric_ed(A):
             acc = acc + phi[j].diff(k) * y[j]
         eqs.append(acc)
     return System(eqs, d + n, n)
+
+
+def circle_line():
+    """x² + y² − 1, a x + b y + c with parameters (a, b, c) (test/monodromy_test.jl:359-362)."""
+    (x, y), (a, b, c) = variables(2, 3)
+    return System([x**2 + y**2 - 1, a * x + b * y + c], 2, 3)

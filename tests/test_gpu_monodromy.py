@@ -67,3 +67,20 @@ def test_monodromy_toric_ed_gpu(gpu_ctx):
     assert g.returncode == "success" and g.nsolutions() == 21
     assert g.new_per_loop == o.new_per_loop
     np.testing.assert_array_equal(g.solutions, o.solutions)
+
+
+def test_verify_solution_completeness_gpu(gpu_ctx):
+    """The trace test (test/monodromy_test.jl:358-384) with every homotopy on the GPU."""
+    import systems
+    from hcgpu.monodromy import verify_solution_completeness
+    F = systems.circle_line()
+    q = [1.0, 2.0, 3.0]
+    ys = np.roots([5, 12, 8])
+    S = np.array([[-3 - 2 * yy, yy] for yy in ys], dtype=complex)
+    tr = gpu_edge_tracker(gpu_ctx)
+    assert verify_solution_completeness(F, S, q, tracker=tr) is True
+    assert verify_solution_completeness(F, S[:1], q, tracker=tr) is False
+    G = sparse_param_system(3, 6, 4, seed=7)
+    r = monodromy_solve(G, seed=11, tracker=tr, target_solutions_count=8)
+    assert verify_solution_completeness(G, r.solutions, r.parameters, tracker=tr) is True
+    assert verify_solution_completeness(G, r.solutions[:7], r.parameters, tracker=tr) is False

@@ -93,3 +93,17 @@ def test_monodromy_toric_ed_21():
     H = ParameterHomotopy(F, r.parameters, q)
     o = oracle.track_batch(H, r.solutions, make_options())
     assert ((o.code == 1) & (~o.singular)).sum() == 21
+
+
+def test_verify_solution_completeness():
+    """test/monodromy_test.jl:358-384: circle ∩ line has 2 solutions; the trace test accepts
+    both and rejects one alone."""
+    from hcgpu.monodromy import verify_solution_completeness
+    F = systems.circle_line()
+    q = [1.0, 2.0, 3.0]
+    # the two intersection points of x² + y² = 1 with x + 2y + 3 = 0
+    ys = np.roots([5, 12, 8])
+    S = np.array([[-3 - 2 * yy, yy] for yy in ys], dtype=complex)
+    assert max(abs(f(list(s) + q)) for s in S for f in F.polys) < 1e-12
+    assert verify_solution_completeness(F, S, q, tracker=oracle_tracker) is True
+    assert verify_solution_completeness(F, S[:1], q, tracker=oracle_tracker) is False
