@@ -129,7 +129,12 @@ def solver_startsolutions(F: System, starts=None, *, start_system="total_degree"
 def solve(F: System, starts=None, **kw):
     """solve(F; start_system = "total_degree", gamma, seed) -> Result, or
     solve(F, starts; start_parameters = p, target_parameters = q) -> Result /
-    [(Result, q_k)] when q is a list of parameter vectors."""
+    [(Result, q_k)] when q is a list of parameter vectors.  A monodromy result as `starts`
+    supplies the start solutions and start parameters (its solutions and parameters,
+    src/monodromy.jl:351, 391; used so in test/monodromy_test.jl:347-356)."""
+    if starts is not None and hasattr(starts, "parameters") and hasattr(starts, "new_per_loop"):
+        kw.setdefault("start_parameters", starts.parameters)
+        starts = starts.solutions
     tp = kw.get("target_parameters")
     many = kw.get("start_parameters") is not None and tp is not None and np.asarray(tp).ndim == 2
     solver, S = solver_startsolutions(F, starts, **kw)

@@ -67,6 +67,11 @@ def test_monodromy_toric_ed_gpu(gpu_ctx):
     assert g.returncode == "success" and g.nsolutions() == 21
     assert g.new_per_loop == o.new_per_loop
     np.testing.assert_array_equal(g.solutions, o.solutions)
+    # parameter homotopy with the monodromy result (test/monodromy_test.jl:347-356)
+    from hcgpu.solve import solve
+    q = (np.random.default_rng(3).standard_normal(4) + 1j * np.random.default_rng(4).standard_normal(4)) / np.sqrt(2)
+    res = solve(F, g, target_parameters=q)
+    assert res.nsolutions() == 21
 
 
 def test_verify_solution_completeness_gpu(gpu_ctx):
