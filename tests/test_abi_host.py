@@ -229,3 +229,19 @@ def test_result_counts_singular_clusters_once():
     assert R.nat_infinity() == 1
     assert R.nfailed() == 1
     assert R.nresults() == 3
+
+
+def test_certifier_compile_errors_are_status_codes():
+    """hcg_certifier_compile_only reports bad systems as error codes with a message."""
+    import ctypes
+    from hcgpu.homotopies import _SysTapes
+    (x, y), _ = variables(2, 0)
+    F = System([x**2 + y**2 - 1, x - y], 2, 0)
+    tapes = _SysTapes(F)
+    st = tapes.struct()
+    L = abi.lib()
+    size = ctypes.c_size_t()
+    assert L.hcg_certifier_compile_only(ctypes.byref(st), 3, ctypes.byref(size)) != 0  # wrong size
+    assert b"" != L.hcg_last_error()
+    assert L.hcg_certifier_compile_only(None, 2, ctypes.byref(size)) != 0
+    assert L.hcg_certifier_compile_only(ctypes.byref(st), 2, ctypes.byref(size)) == 0 and size.value > 0

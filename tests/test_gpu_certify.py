@@ -126,3 +126,18 @@ def test_certify_solve_result(gpu_ctx):
     assert res.nnonsingular() == 18
     r = certify(F, res, ctx=gpu_ctx)
     assert (r.ncertified(), r.nreal_certified(), r.ndistinct_certified()) == (18, 4, 18)
+
+
+def test_certify_argument_errors(gpu_ctx):
+    """Bad arguments are rejected before any launch, with a message."""
+    from hcgpu.abi import HcgpuError
+    G = critical_points_system()
+    cert = gpu_ctx.certifier(G)
+    S = np.zeros((3, 3), dtype=complex)
+    with pytest.raises(ValueError):
+        cert.certify(S)  # parameters missing
+    with pytest.raises(HcgpuError, match="out of range"):
+        cert.certify(S, [[1.0, 2.0]], group_of=[0, 1, 0])
+    with pytest.raises(HcgpuError, match="not in its interval"):
+        cert.certify(S[:1], [1.0, 2.0], params_ival=[[2.0, 3.0, 0.0, 0.0], [2.0, 2.0, 0.0, 0.0]])
+    assert len(cert.certify(S[:0], [1.0, 2.0]).certified) == 0
