@@ -242,6 +242,10 @@ def _candidates(X, n) -> np.ndarray:
         return np.asarray(S, dtype=np.complex128).reshape(-1, n)
     if hasattr(X, "solutions") and isinstance(getattr(X, "solutions"), np.ndarray):  # MonodromyResult
         return np.asarray(X.solutions, dtype=np.complex128).reshape(-1, n)
+    if hasattr(X, "solution"):  # one PathResult
+        X = [X]
+    if isinstance(X, (list, tuple)) and X and hasattr(X[0], "solution"):  # PathResults
+        X = [r.solution for r in X]
     return np.asarray(X, dtype=np.complex128).reshape(-1, n)
 
 
@@ -249,8 +253,8 @@ def certify(F: System, X, p: Optional[Sequence] = None, *, target_parameters: Op
             refine: bool = True, extended: bool = True, check_distinct: bool = True, ctx=None,
             certifier=None) -> CertificationResult:
     """certify(F, solutions, [p]): certify the candidates X (an array of points, a Result,
-    or a MonodromyResult, whose parameters are used when p is not given) as zeros of
-    F(x; p), on the GPU in one launch."""
+    PathResults, or a MonodromyResult, whose parameters are used when p is not given) as
+    zeros of F(x; p), on the GPU in one launch (src/certification.jl:1411-1560)."""
     n = F.nvars
     if len(F) != n:
         raise ValueError("We can only certify solutions to square systems.")

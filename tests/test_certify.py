@@ -369,6 +369,18 @@ def test_distinct_certified_solutions_incremental_api():
     assert len(left.solutions()) == 2 and left.nduplicates() == 1
 
 
+def test_certify_accepts_path_results():
+    """certify(F, ::Vector{PathResult}) and certify(F, ::PathResult) (src/certification.jl:1479-1506)."""
+    from hcgpu.certify import certify
+    from hcgpu.result import PathResult
+    F = simple1()
+    S = solve_nonsingular(F)[:3]
+    prs = [PathResult(solution=x, return_code="success") for x in S]
+    r = certify(F, prs, certifier=OracleCertifier(F))
+    assert r.ncertified() == 3
+    assert certify(F, prs[0], certifier=OracleCertifier(F)).ncertified() == 1
+
+
 def test_positive():
     (x, y), _ = variables(2, 0)
     F = System([x**2 + y**2 - 1, x - y], 2, 0)
