@@ -213,3 +213,17 @@ def circle_line():
     """x² + y² − 1, a x + b y + c with parameters (a, b, c) (test/monodromy_test.jl:359-362)."""
     (x, y), (a, b, c) = variables(2, 3)
     return System([x**2 + y**2 - 1, a * x + b * y + c], 2, 3)
+
+
+def quotient_tapes():
+    """Hand-written tapes of F = [x0 / x1 - 2, x0 + x1 - 3] (zero at (2, 1)): the eval tape
+    divides (OP_DIV), the Jacobian tape inverts and negates (∂f1/∂x0 = 1/x1,
+    ∂f1/∂x1 = -(x0/x1)/x1).  Slots: constants 2, 3, 1 (1..3), x0 = 4, x1 = 5."""
+    from hcgpu.tape import Tape
+    OP_INV, OP_NEG, OP_ADD, OP_DIV, OP_MUL, OP_SUB = 7, 10, 18, 19, 20, 21
+    consts = [2.0, 3.0, 1.0]
+    ev = [(4, 5, 0, 0, OP_DIV, 6), (6, 1, 0, 0, OP_SUB, 7), (4, 5, 0, 0, OP_ADD, 8), (8, 2, 0, 0, OP_SUB, 9)]
+    jac = ev + [(5, 0, 0, 0, OP_INV, 10), (6, 10, 0, 0, OP_MUL, 11), (11, 0, 0, 0, OP_NEG, 12)]
+    u = [(1, 7), (2, 9)]
+    U = [(1, 10), (3, 12), (2, 3), (4, 3)]  # column-major: (1,1), (1,2), (2,1), (2,2)
+    return Tape(ev, consts, 0, 2, u, [], 2), Tape(jac, consts, 0, 2, u, U, 2)

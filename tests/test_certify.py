@@ -381,6 +381,29 @@ def test_certify_accepts_path_results():
     assert certify(F, prs[0], certifier=OracleCertifier(F)).ncertified() == 1
 
 
+def _quotient_system():
+    from hcgpu.abi import hcg_system
+    import ctypes
+    ev, ej = systems.quotient_tapes()
+    st = hcg_system()
+    st.eval_jac = ctypes.pointer(ej.struct())
+    st.eval = ctypes.pointer(ev.struct())
+    return st, (ev, ej)
+
+
+def test_certify_with_quotients():
+    """Interval division and inversion (src/interval_arithmetic.jl:196-227, 356-366) inside the
+    SLP: the zero (2, 1) of [x0/x1 - 2, x0 + x1 - 3] certifies (real), a perturbed start is
+    refined onto it, and a point with x1 near 0 does not certify."""
+    st, keep = _quotient_system()
+    S = np.array([[2.0, 1.0], [2.0 + 1e-9, 1.0 - 1e-9j], [1.0, 1e-300]], dtype=complex)
+    o = oracle.certify(st, 2, S, real_system=True)
+    assert list(o.certified) == [1, 1, 0] and list(o.real[:2]) == [1, 1]
+    for k in range(2):
+        b = o.box1[k]
+        assert b[0, 0] <= 2.0 <= b[0, 1] and b[1, 0] <= 1.0 <= b[1, 1]
+
+
 def test_positive():
     (x, y), _ = variables(2, 0)
     F = System([x**2 + y**2 - 1, x - y], 2, 0)

@@ -141,3 +141,28 @@ def test_certify_argument_errors(gpu_ctx):
     with pytest.raises(HcgpuError, match="not in its interval"):
         cert.certify(S[:1], [1.0, 2.0], params_ival=[[2.0, 3.0, 0.0, 0.0], [2.0, 2.0, 0.0, 0.0]])
     assert len(cert.certify(S[:0], [1.0, 2.0]).certified) == 0
+
+
+def test_certify_with_quotients_bitwise_equals_oracle(gpu_ctx):
+    """The interval quotient path (ci_div, ci_inv) on the GPU, on hand-written tapes."""
+    import ctypes
+    from hcgpu import abi
+    from hcgpu.gpu import _CERT_DTYPE, _dp
+    from test_certify import _quotient_system
+    st, keep = _quotient_system()
+    h = ctypes.c_void_p()
+    abi.check(abi.lib().hcg_certifier_create(gpu_ctx.handle, ctypes.byref(st), 2, ctypes.byref(h)))
+    try:
+        S = np.ascontiguousarray(np.array([[2.0, 1.0], [2.0 + 1e-9, 1.0 - 1e-9j], [1.0, 1e-300]], dtype=complex))
+        rec = np.zeros(3, dtype=_CERT_DTYPE)
+        b1, b0, ref = np.zeros((3, 2, 4)), np.zeros((3, 2, 4)), np.zeros((3, 2), dtype=complex)
+        abi.check(abi.lib().hcg_certify(gpu_ctx.handle, h, 3, _dp(S), 0, None, None, None, None, 1, 1, 1,
+                                        rec.ctypes.data_as(ctypes.POINTER(abi.hcg_certificate)), _dp(b1), _dp(b0),
+                                        _dp(ref)))
+    finally:
+        abi.lib().hcg_program_destroy(h)
+    o = oracle.certify(st, 2, S, real_system=True)
+    for f in FIELDS:
+        np.testing.assert_array_equal(rec[f], getattr(o, f), err_msg=f)
+    np.testing.assert_array_equal(b1, o.box1)
+    assert list(rec["certified"]) == [1, 1, 0]
