@@ -119,8 +119,21 @@ def solver_startsolutions(F: System, starts=None, *, start_system="total_degree"
         H = ParameterHomotopy(F, start_parameters, q0, lift_affine=lift_affine)
         T = EndgameTracker(H, tracker_options, endgame_options, **gpu)
         return Solver(T, seed=seed), starts
+    if start_system == "polyhedral":
+        # src/solve.jl: solver_startsolutions(F; start_system = :polyhedral, only_non_zero):
+        # the toric stage runs here (one launch per mixed cell); the returned solver tracks
+        # the coefficient homotopy from the generic system to F, with the endgame
+        from .monodromy import gpu_edge_tracker
+        from .polyhedral import polyhedral
+        only_non_zero = bool(gpu.pop("only_non_zero", False) or gpu.pop("only_torus", False))
+        P = polyhedral(F if target_parameters is None else F.fix_parameters(target_parameters),
+                       only_non_zero=only_non_zero, seed=seed)
+        ctx = gpu.get("ctx")
+        S, _ = P.start_solutions(tracker=gpu_edge_tracker(ctx))
+        T = EndgameTracker(P.homotopy(), tracker_options, endgame_options, **gpu)
+        return Solver(T, seed=seed, start_system="polyhedral"), S
     if start_system != "total_degree":
-        raise NotImplementedError("only start_system='total_degree' (polyhedral start systems are out of scope)")
+        raise NotImplementedError(f"start_system={start_system!r}: use 'total_degree' or 'polyhedral'")
     H, S = total_degree(F, gamma=gamma, seed=seed, target_parameters=target_parameters)
     T = EndgameTracker(H, tracker_options, endgame_options, **gpu)
     return Solver(T, seed=seed, start_system="total_degree"), S

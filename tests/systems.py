@@ -227,3 +227,12 @@ def quotient_tapes():
     u = [(1, 7), (2, 9)]
     U = [(1, 10), (3, 12), (2, 3), (4, 3)]  # column-major: (1,1), (1,2), (2,1), (2,2)
     return Tape(ev, consts, 0, 2, u, [], 2), Tape(jac, consts, 0, 2, u, U, 2)
+
+
+def polyhedral_cases():
+    """(name, F, only_non_zero, seed) of the polyhedral runs on the GPU: f of
+    test/polyhedral_test.jl:3-11 (8 start solutions, 6 finite; 3 in the torus) and
+    cyclic-5 (70 solutions, :39-42)."""
+    (x, y), _ = variables(2)
+    f = System([2 * y + 3 * y ** 2 - x * y ** 3, x + 4 * x ** 2 - 2 * x ** 3 * y], 2)
+    return [("f_affine", f, False, 1), ("f_torus", f, True, 1), ("cyclic5", cyclic(5), True, 3)]
