@@ -125,7 +125,8 @@ def solver_startsolutions(F: System, starts=None, *, start_system="total_degree"
         # the coefficient homotopy from the generic system to F, with the endgame
         from .monodromy import gpu_edge_tracker
         from .polyhedral import polyhedral
-        only_non_zero = bool(gpu.pop("only_non_zero", False) or gpu.pop("only_torus", False))
+        only_non_zero = bool(gpu.pop("only_non_zero", False))
+        only_non_zero = bool(gpu.pop("only_torus", only_non_zero))
         P = polyhedral(F if target_parameters is None else F.fix_parameters(target_parameters),
                        only_non_zero=only_non_zero, seed=seed)
         ctx = gpu.get("ctx")
