@@ -44,6 +44,8 @@ def lib():
                                           P(ctypes.c_int)]
         L.orc_track_plain.argtypes = [P(hcg_homotopy_desc), P(hcg_options), P(D), P(D), P(D), D, D, P(D),
                                       P(ctypes.c_int32), P(D)]
+        L.orc_predictor_probe.argtypes = [P(hcg_homotopy_desc), P(hcg_options), P(D), P(D), ctypes.c_int,
+                                          ctypes.c_int, P(D), P(D), P(D), D, D, P(D), P(D), P(D), P(D)]
         L.orc_sys_eval.argtypes = [P(hcg_tape), P(D), P(D), ctypes.c_int, P(D), P(D)]
         L.orc_sys_taylor.argtypes = [P(hcg_tape), ctypes.c_int, P(D), P(D), P(D), ctypes.c_int, P(D)]
         L.orc_sys_eval_dd.argtypes = [P(hcg_tape), P(D), P(D), P(D), ctypes.c_int, P(D), P(D)]
@@ -170,6 +172,36 @@ def track_plain(H, x1, t1=1.0, t0=0.0, opts: hcg_options | None = None, p=None, 
     return dict(return_code=TRACKER_CODES[io[0]], solution=xo, accepted_steps=io[1], rejected_steps=io[2],
                 extended_precision=bool(io[3]), extended_precision_used=bool(io[4]), t=complex(do[0], do[1]),
                 accuracy=do[2], omega=do[3], mu=do[4], tau=do[5])
+
+
+def predictor_probe(H, xs, ts, dt, winding=1, w=None, p=None, q=None, opts: hcg_options | None = None):
+    """The tracker's predictor on given path points (orc_predictor_probe): predict! from each
+    point to the next, update! at every point, then predict! by dt from the last one.
+    Returns dict(tx = 4 x n Taylor coefficients x⁰..x³ after the last update, xh = the final
+    prediction, xhats = the prediction made before each update (row 0 unused), and per update
+    trust_region, local_error, cond_H_xd, hermite, order)."""
+    opts = opts or default_options()
+    xs = np.ascontiguousarray(np.asarray(xs, dtype=np.complex128).reshape(len(ts), -1))
+    npts, n = xs.shape
+    tsc = _c(np.asarray(ts, dtype=np.complex128))
+    if p is None and hasattr(H, "p"):
+        p, q = H.p, H.q[0]
+    pp = _c(p) if p is not None else np.zeros(1, dtype=np.complex128)
+    qq = _c(q) if q is not None else np.zeros(1, dtype=np.complex128)
+    ww = np.ascontiguousarray(np.asarray(w, dtype=np.float64)) if w is not None else None
+    tx = np.zeros(4 * n, dtype=np.complex128)
+    xh = np.zeros(n, dtype=np.complex128)
+    xhats = np.zeros(npts * n, dtype=np.complex128)
+    scal = np.zeros(5 * npts)
+    dt = complex(dt)
+    rc = lib().orc_predictor_probe(ctypes.byref(H.desc()), ctypes.byref(opts), _dp(pp), _dp(qq), int(winding), npts,
+                                   _dp(xs), _dp(tsc), _dp(ww) if ww is not None else ctypes.POINTER(ctypes.c_double)(),
+                                   dt.real, dt.imag, _dp(tx), _dp(xh), _dp(xhats), _dp(scal))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    sc = scal.reshape(npts, 5)
+    return dict(tx=tx.reshape(4, n), xh=xh, xhats=xhats.reshape(npts, n), trust_region=sc[:, 0],
+                local_error=sc[:, 1], cond_H_xd=sc[:, 2], hermite=sc[:, 3].astype(int), order=sc[:, 4].astype(int))
 
 
 def sys_eval(tape, x, p=(), n=None):

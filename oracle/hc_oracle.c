@@ -2111,6 +2111,62 @@ int orc_track_plain(const orc_homotopy_desc* hd, const orc_options* o, const dou
     return 0;
 }
 
+/* Predictor probe for the oracle's pinning tests: runs the tracker's predictor sequence
+ * on given points of a path, exactly as step! uses it (src/tracker.jl:851-926):
+ *   for k = 0..npts-1:  [k > 0: predict!(x̂, pred, H, x_{k-1}, t_{k-1}, t_k - t_{k-1})]
+ *                       J <- factorization of H_x(x_k, t_k)  (the corrector leaves an unscaled one)
+ *                       update!(pred, H, x_k, t_k, J, norm, k > 0 ? x̂ : nothing)
+ *   then predict!(out, pred, H, x_last, t_last, dt).
+ * xs: npts*n complex, ts: npts complex, w: n weights of the WeightedNorm (NULL: all 1).
+ * Outputs: tx: 4n complex (x⁰..x³ after the last update), xh: n complex (final prediction),
+ * xhats: npts*n complex (the prediction made before update k; zero for k = 0),
+ * scal: npts*5 doubles per update (trust_region, local_error, cond_H_ẋ, hermite, order). */
+int orc_predictor_probe(const orc_homotopy_desc* hd, const orc_options* o, const double* p, const double* q,
+                        int winding, int npts, const double* xs, const double* ts, const double* w, double dt_re,
+                        double dt_im, double* tx, double* xh, double* xhats, double* scal) {
+    if (check_desc(hd) || npts < 1 || winding < 1) return -1;
+    orc_tracker T;
+    tracker_create(&T, hd, o);
+    int n = hd->n;
+    if (hd->kind == 1) {
+        int np = T.H.npar;
+        cplx *pp = cvec(np + 1), *qq = cvec(np + 1);
+        for (int i = 0; i < np; i++) { pp[i] = C(p[2 * i], p[2 * i + 1]); qq[i] = C(q[2 * i], q[2 * i + 1]); }
+        H_set_params(&T.H, pp, qq);
+        free(pp); free(qq);
+    }
+    if (w) for (int i = 0; i < n; i++) T.w[i] = w[i];
+    pred_init(&T);
+    T.winding = winding;
+    cplx *x = cvec(n), *xp = cvec(n), *xhat = cvec(n);
+    cplx tprev = C(NAN, 0);
+    for (int k = 0; k < npts; k++) {
+        cplx t = C(ts[2 * k], ts[2 * k + 1]);
+        for (int i = 0; i < n; i++) x[i] = C(xs[2 * (k * n + i)], xs[2 * (k * n + i) + 1]);
+        if (k > 0) predict(&T, xhat, xp, tprev, csub(t, tprev));
+        for (int i = 0; i < n; i++) {
+            xhats[2 * (k * n + i)] = k > 0 ? xhat[i].re : 0.0;
+            xhats[2 * (k * n + i) + 1] = k > 0 ? xhat[i].im : 0.0;
+        }
+        H_eval_jac(&T.H, x, t, T.r, T.J.A);
+        ws_updated(&T.J);
+        ws_factorize(&T.J);
+        pred_update(&T, x, t, k > 0 ? xhat : NULL);
+        scal[5 * k] = T.trust_region; scal[5 * k + 1] = T.local_error; scal[5 * k + 2] = T.cond_H_xd;
+        scal[5 * k + 3] = T.pred_hermite; scal[5 * k + 4] = T.pred_order;
+        memcpy(xp, x, sizeof(cplx) * n);
+        tprev = t;
+    }
+    const cplx* txs[4] = {T.tx0, T.tx1, T.tx2, T.tx3};
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < n; i++) { tx[2 * (k * n + i)] = txs[k][i].re; tx[2 * (k * n + i) + 1] = txs[k][i].im; }
+    predict(&T, xhat, xp, tprev, C(dt_re, dt_im));
+    for (int i = 0; i < n; i++) { xh[2 * i] = xhat[i].re; xh[2 * i + 1] = xhat[i].im; }
+    free(x); free(xp); free(xhat);
+    tracker_destroy(&T);
+    return 0;
+}
+
 /* ---- unit-level entry points used by the oracle's own pinning tests ---- */
 int orc_sys_eval(const orc_tape* ej, const double* x, const double* p, int n, double* u, double* U) {
     if (orc_tape_check(ej)) return -1;
