@@ -19,10 +19,38 @@ from .polys import Poly, System, variables
 from .tape import Tape, tapes
 
 
+class TapeSystem:
+    """A square system given directly by its two straight-line programs, the form in which
+    ModelKit hands a compiled system over: the `evaluate_and_jacobian!` tape (u and U
+    assignments) and the `evaluate!` tape (u only; also used for `taylor!` and the
+    double-double residual), as `jacobian_instruction_sequence` / `instruction_sequence`
+    produce them (src/model_kit/instruction_sequence.jl:1-27, 256-272).  Any OpType the C ABI
+    accepts may appear, including the fused ADD3/ADD4/MUL3/MUL4/MULADD/MULSUB/SUBMUL/
+    MULMULADD/MULMULSUB and the inverse ops (src/model_kit/operations.jl:183-248)."""
+
+    def __init__(self, eval_jac: Tape, eval_: Tape):
+        if eval_jac.nvars != eval_.nvars or eval_jac.nparams != eval_.nparams:
+            raise ValueError("the two tapes disagree on variables / parameters")
+        if eval_.output_dim != eval_.nvars or eval_jac.output_dim != eval_.nvars:
+            raise ValueError("only square systems")
+        self.tapes = (eval_jac, eval_)
+        self.nvars = eval_.nvars
+        self.nparams = eval_.nparams
+
+    def __len__(self):
+        return self.nvars
+
+    @property
+    def n(self):
+        return self.nvars
+
+
 class _SysTapes:
     def __init__(self, system: System, fixed_params: Optional[Sequence[complex]] = None):
         self.system = system
-        if hasattr(system, "exprs"):
+        if isinstance(system, TapeSystem):
+            self.eval_jac, self.eval = system.tapes
+        elif hasattr(system, "exprs"):
             from .expr import expr_tapes
             self.eval_jac, self.eval = expr_tapes(system)
         else:

@@ -25,6 +25,7 @@ the torus (src/polyhedral.jl:200-225).
 from __future__ import annotations
 
 import itertools
+import math
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -34,6 +35,13 @@ from .homotopies import ParameterHomotopy
 from .polys import Poly, System
 
 _CHUNK = 1 << 16
+# exact enumeration visits every tuple of one point pair per support; beyond this many tuples
+# it is refused with an error instead of running for hours
+MAX_PAIR_TUPLES = 50_000_000
+
+
+class EnumerationTooLarge(RuntimeError):
+    """The supports have too many point-pair tuples for the exact mixed-cell enumeration."""
 
 
 def _supports(F: System, only_non_zero: bool):
@@ -74,7 +82,11 @@ def mixed_cells(A: Sequence[np.ndarray], lifting: Sequence[np.ndarray]) -> List[
     if any(len(p) == 0 for p in P):
         return []
     sizes = [len(p) for p in P]
-    total = int(np.prod(sizes))
+    total = math.prod(sizes)  # Python integers: no int64 overflow for large supports
+    if total > MAX_PAIR_TUPLES:
+        raise EnumerationTooLarge(
+            f"mixed cells: {total:.3g} point-pair tuples exceed the exact enumeration's limit of "
+            f"{MAX_PAIR_TUPLES:.3g}; use start_system='total_degree' for this system")
     cells = []
     for lo in range(0, total, _CHUNK):
         idx = np.stack(np.unravel_index(np.arange(lo, min(total, lo + _CHUNK)), sizes), axis=1)
@@ -157,22 +169,35 @@ class PolyhedralStart:
     """Start data of the polyhedral homotopy of F: lifting, mixed cells, generic
     coefficients, and the two stages (toric per cell, then coefficients)."""
 
-    def __init__(self, F: System, only_non_zero: bool = False, seed: Optional[int] = None, lifting_range: int = 32):
+    def __init__(self, F: System, only_non_zero: bool = False, seed: Optional[int] = None, lifting_range: int = 32,
+                 liftings: int = 8):
         self.F = F
         self.n = F.nvars
         self.only_non_zero = only_non_zero
         rng = np.random.default_rng(seed)
         self.A, self.target_coefficients = _supports(F, only_non_zero)
-        for attempt in range(60):  # small liftings keep the powers of s low; widen on ties
-            r = lifting_range * 4 ** (attempt // 10)
-            self.lifting = [rng.integers(0, r, len(a)).astype(np.int64) for a in self.A]
+        # Among `liftings` generic liftings keep the one whose toric homotopies have the lowest
+        # largest power of s: high powers make the toric paths steep near s = 1 (the reason the
+        # reference splits such paths in two stages, src/polyhedral.jl:436-490).
+        best = None
+        attempt = 0
+        while (best is None or attempt < liftings) and attempt < 60 + liftings:
+            r = lifting_range * 4 ** (attempt // 10)  # small liftings keep the powers low; widen on ties
+            attempt += 1
+            lifting = [rng.integers(0, r, len(a)).astype(np.int64) for a in self.A]
             try:
-                self.cells = mixed_cells(self.A, self.lifting)
-                break
+                cells = mixed_cells(self.A, lifting)
+            except EnumerationTooLarge:
+                raise
             except ValueError:
                 continue
-        else:
+            top = max((int(max(e.max() for e in c.s_exponents)) for c in cells), default=0)
+            if best is None or top < best[0]:
+                best = (top, lifting, cells)
+        if best is None:
             raise RuntimeError("no generic lifting found")
+        self.max_power, self.lifting, self.cells = best
+        self.failed = []  # toric paths that did not reach s = 1 (filled by start_solutions)
         self.start_coefficients = [(rng.standard_normal(len(a)) + 1j * rng.standard_normal(len(a))) / np.sqrt(2)
                                    for a in self.A]
 
@@ -215,32 +240,56 @@ class PolyhedralStart:
 
     def start_solutions(self, tracker=None) -> Tuple[np.ndarray, int]:
         """Stage 1 for every cell (one launch per cell): solutions of G(x; c) and the number
-        of toric paths that failed."""
-        from .monodromy import gpu_edge_tracker
-        tracker = tracker or gpu_edge_tracker()
-        out, failed = [], 0
+        of toric paths that failed.  `tracker(H, starts, p, q)` returns (codes, endpoints) or
+        (codes, endpoints, final t); the failed paths are kept in self.failed as
+        (start solution, last point, t) for their `polyhedral_failed` path results."""
+        tracker = tracker or _gpu_toric_tracker()
+        out = []
+        self.failed = []
         for cell in self.cells:
             X = binomial_solutions(cell, self.A, self.start_coefficients)
             H = ParameterHomotopy(self.toric_system(cell), [0.0], [1.0])
-            codes, sols = tracker(H, X, np.array([0.0 + 0j]), np.array([1.0 + 0j]))
-            ok = np.asarray(codes) == 1
-            failed += int((~ok).sum())
+            r = tracker(H, X, np.array([0.0 + 0j]), np.array([1.0 + 0j]))
+            codes, sols = np.asarray(r[0]), r[1]
+            ts = np.asarray(r[2]) if len(r) > 2 else np.full(len(X), np.nan)
+            ok = codes == 1
+            for k in np.flatnonzero(~ok):
+                self.failed.append((X[k], sols[k], float(ts[k])))
             out.append(sols[ok])
         S = np.vstack(out) if out else np.zeros((0, self.n), dtype=np.complex128)
-        return S, failed
+        return S, len(self.failed)
 
     def homotopy(self) -> ParameterHomotopy:
         """Stage 2: the coefficient homotopy from G(x; c) to F."""
         return ParameterHomotopy(self.coefficient_system(), self.start_parameters(), self.target_parameters())
 
 
-def polyhedral(F: System, only_non_zero: bool = False, only_torus: Optional[bool] = None,
+def _gpu_toric_tracker(ctx=None):
+    """Toric stage on the GPU: (codes, endpoints, final t) of one launch."""
+    from .gpu import default_context, track_batch
+    from .solve import make_options
+    ctx = ctx or default_context()
+
+    def track(H, starts, p, q):
+        out = track_batch(ctx.program(H), starts, make_options(), target_groups=q[None, :], start_params=p)
+        return out.code, out.solution, 1.0 - out.t  # s = 1 - t runs from 0 to 1
+
+    return track
+
+
+def _non_zero(only_non_zero: Optional[bool], only_torus: Optional[bool]) -> bool:
+    """only_torus is only the default of only_non_zero (src/polyhedral.jl:303-304, 364-365):
+    an explicitly given only_non_zero wins."""
+    if only_non_zero is not None:
+        return bool(only_non_zero)
+    return bool(only_torus) if only_torus is not None else False
+
+
+def polyhedral(F: System, only_non_zero: Optional[bool] = None, only_torus: Optional[bool] = None,
                seed: Optional[int] = None) -> PolyhedralStart:
     """polyhedral(F; only_non_zero) (src/polyhedral.jl:186-260); `only_torus` is the older
-    name of the same switch (test/polyhedral_test.jl:6-10)."""
-    if only_torus is not None:
-        only_non_zero = only_torus
-    return PolyhedralStart(F, only_non_zero=only_non_zero, seed=seed)
+    name of the same switch and only its default (test/polyhedral_test.jl:6-10)."""
+    return PolyhedralStart(F, only_non_zero=_non_zero(only_non_zero, only_torus), seed=seed)
 
 
 def mixed_volume(F: System, seed: Optional[int] = 0) -> int:
@@ -248,12 +297,10 @@ def mixed_volume(F: System, seed: Optional[int] = 0) -> int:
     return PolyhedralStart(F, only_non_zero=True, seed=seed).mixed_volume
 
 
-def paths_to_track(F: System, start_system: str = "polyhedral", only_non_zero: bool = False,
+def paths_to_track(F: System, start_system: str = "polyhedral", only_non_zero: Optional[bool] = None,
                    only_torus: Optional[bool] = None) -> int:
     """paths_to_track (src/polyhedral.jl:300, src/total_degree.jl:481): Bezout number, or the mixed volume of the supports
     (with the origin added unless only_non_zero)."""
     if start_system == "total_degree":
-        return int(np.prod(F.degrees()))
-    if only_torus is not None:
-        only_non_zero = only_torus
-    return PolyhedralStart(F, only_non_zero=only_non_zero, seed=0).mixed_volume
+        return math.prod(F.degrees())
+    return PolyhedralStart(F, only_non_zero=_non_zero(only_non_zero, only_torus), seed=0).mixed_volume

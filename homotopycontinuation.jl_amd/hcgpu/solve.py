@@ -75,20 +75,40 @@ def track(tracker: EndgameTracker, x, t1: float = 1.0, path_number: Optional[int
     return r
 
 
+def polyhedral_failed_results(failed, first_path_number=1):
+    """PathResults with return code `polyhedral_failed` for toric-stage paths that did not reach
+    the generic system, given as (start solution, last point, s) (src/polyhedral.jl:491-511)."""
+    out = []
+    for k, (start, point, s) in enumerate(failed):
+        out.append(PathResult(return_code="polyhedral_failed", solution=np.array(point), t=float(s),
+                              accuracy=float("nan"), residual=float("nan"), singular=False,
+                              condition_jacobian=float("nan"), winding_number=None, extended_precision=False,
+                              path_number=first_path_number + k, start_solution=np.array(start),
+                              last_path_point=(np.array(point), float(s)), valuation=None, ω=float("nan"),
+                              μ=float("nan"), accepted_steps=0, rejected_steps=0, steps_eg=0,
+                              extended_precision_used=False, newton_iters=0))
+    return out
+
+
 class Solver:
     """Solver(tracker; seed, start_system), src/solve.jl:41-51."""
 
-    def __init__(self, tracker: EndgameTracker, seed=None, start_system=None):
+    def __init__(self, tracker: EndgameTracker, seed=None, start_system=None, failed_starts=()):
         self.tracker = tracker
         self.seed = seed
         self.start_system = start_system
+        # paths that failed before reaching this tracker's start system (the toric stage of a
+        # polyhedral start system): reported as `polyhedral_failed` path results
+        self.failed_starts = list(failed_starts)
 
     def solve(self, starts, target_parameters=None):
         starts = np.asarray(starts, dtype=np.complex128)
         H = self.tracker.homotopy
         if target_parameters is None:
             out = self.tracker.track_batch(starts)
-            return Result(path_results_from_batch(out, starts), seed=self.seed, start_system=self.start_system)
+            prs = path_results_from_batch(out, starts)
+            prs += polyhedral_failed_results(self.failed_starts, len(prs) + 1)
+            return Result(prs, seed=self.seed, start_system=self.start_system)
         # many parameters: every (target, start) pair in one launch
         Qorig = np.asarray(target_parameters, dtype=np.complex128).reshape(-1, H.original.nparams)
         Q = H.target_matrix(Qorig)
@@ -123,16 +143,14 @@ def solver_startsolutions(F: System, starts=None, *, start_system="total_degree"
         # src/solve.jl: solver_startsolutions(F; start_system = :polyhedral, only_non_zero):
         # the toric stage runs here (one launch per mixed cell); the returned solver tracks
         # the coefficient homotopy from the generic system to F, with the endgame
-        from .monodromy import gpu_edge_tracker
         from .polyhedral import polyhedral
-        only_non_zero = bool(gpu.pop("only_non_zero", False))
-        only_non_zero = bool(gpu.pop("only_torus", only_non_zero))
+        from .polyhedral import _gpu_toric_tracker
         P = polyhedral(F if target_parameters is None else F.fix_parameters(target_parameters),
-                       only_non_zero=only_non_zero, seed=seed)
+                       only_non_zero=gpu.pop("only_non_zero", None), only_torus=gpu.pop("only_torus", None), seed=seed)
         ctx = gpu.get("ctx")
-        S, _ = P.start_solutions(tracker=gpu_edge_tracker(ctx))
+        S, _ = P.start_solutions(tracker=_gpu_toric_tracker(ctx))
         T = EndgameTracker(P.homotopy(), tracker_options, endgame_options, **gpu)
-        return Solver(T, seed=seed, start_system="polyhedral"), S
+        return Solver(T, seed=seed, start_system="polyhedral", failed_starts=P.failed), S
     if start_system != "total_degree":
         raise NotImplementedError(f"start_system={start_system!r}: use 'total_degree' or 'polyhedral'")
     H, S = total_degree(F, gamma=gamma, seed=seed, target_parameters=target_parameters)

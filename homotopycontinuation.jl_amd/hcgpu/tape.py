@@ -40,6 +40,8 @@ ARITY = {
 }
 
 _ARITY_OF = {OP[k]: v for k, v in ARITY.items()}
+# powers of one symbol from this exponent up are one OP_POW_INT instead of a chain of products
+POW_INT_MIN = 5
 
 
 class hcg_tape(ctypes.Structure):
@@ -147,6 +149,20 @@ class _Builder:
         if deg == 1:
             i = nz[0]
             ref = ("x", i) if i < self.nvars else ("p", i - self.nvars)
+        elif e[nz[-1]] >= POW_INT_MIN:
+            # a high power of one symbol is one OP_POW_INT (power by squaring), as ModelKit
+            # lowers x^k (pow!, src/model_kit/intermediate_representation.jl:224-241)
+            last = nz[-1]
+            unit = [0] * len(e)
+            unit[last] = 1
+            if len(nz) == 1:
+                ref = self.emit("POW_INT", self.monomial(tuple(unit)), e[last])
+            else:
+                rest = list(e)
+                rest[last] = 0
+                power = list(unit)
+                power[last] = e[last]
+                ref = self.emit("MUL", self.monomial(tuple(rest)), self.monomial(tuple(power)))
         elif len(nz) == 1 and e[nz[0]] == 2:
             base = list(e)
             base[nz[0]] = 1
@@ -267,7 +283,7 @@ class _Builder:
 
         rows = []
         for k, (op, args) in enumerate(self.instrs):
-            a = [slot(r) for r in args]
+            a = [r if isinstance(r, int) else slot(r) for r in args]
             while len(a) < 4:
                 a.append(a[-1])
             rows.append(a + [OP[op], base + k + 1])

@@ -236,3 +236,199 @@ def polyhedral_cases():
     (x, y), _ = variables(2)
     f = System([2 * y + 3 * y ** 2 - x * y ** 3, x + 4 * x ** 2 - 2 * x ** 3 * y], 2)
     return [("f_affine", f, False, 1), ("f_torus", f, True, 1), ("cyclic5", cyclic(5), True, 3)]
+
+
+# ------------------------------------------------------------------------------------------
+# ModelKit-shaped tapes written by hand: every OpType the C ABI accepts.
+# ------------------------------------------------------------------------------------------
+class HandTape:
+    """Builds an InstructionSequence by hand (slots: constants, parameters, variables, then
+    instruction outputs, 1-based as ModelKit's) and its Jacobian tape by forward-mode
+    differentiation of the same instructions (each derivative written with ADD, SUB, NEG,
+    MUL, DIV, INV, INVSQR and constants)."""
+
+    def __init__(self, nvars, nparams):
+        from hcgpu.tape import OP
+        self.OP = OP
+        self.nvars, self.nparams = nvars, nparams
+        self.consts = []
+        self.ins = []  # (op name, args: refs or int exponent)
+
+    def c(self, v):
+        self.consts.append(complex(v))
+        return ("c", len(self.consts) - 1)
+
+    def x(self, i):
+        return ("x", i)
+
+    def p(self, i):
+        return ("p", i)
+
+    def op(self, name, *args):
+        self.ins.append((name, args))
+        return ("i", len(self.ins) - 1)
+
+    # forward-mode derivative of every instruction output with respect to x_v
+    def _jacobian_instructions(self, outputs):
+        base = list(self.ins)
+        d = {}  # (ref, v) -> derivative ref, absent = structural zero
+
+        def D(ref, v):
+            if ref[0] == "x":
+                return ONE if ref[1] == v else None
+            if ref[0] in ("c", "p"):
+                return None
+            return d.get((ref, v))
+
+        def mul(a, b):
+            if a is None or b is None:
+                return None
+            if a == ONE:
+                return b
+            if b == ONE:
+                return a
+            return self.op("MUL", a, b)
+
+        def add(a, b):
+            if a is None:
+                return b
+            if b is None:
+                return a
+            return self.op("ADD", a, b)
+
+        def neg(a):
+            return None if a is None else self.op("NEG", a)
+
+        ONE = self.c(1.0)
+        for k, (name, args) in enumerate(base):
+            out = ("i", k)
+            refs = [a for a in args if isinstance(a, tuple)]
+            for v in range(self.nvars):
+                ds = [D(a, v) for a in refs]
+                if all(e is None for e in ds):
+                    continue
+                a = refs
+                if name in ("ADD", "ADD3", "ADD4", "IDENTITY"):
+                    r = None
+                    for e in ds:
+                        r = add(r, e)
+                elif name == "SUB":
+                    r = add(ds[0], neg(ds[1]))
+                elif name == "NEG":
+                    r = neg(ds[0])
+                elif name in ("MUL", "MUL3", "MUL4"):
+                    r = None
+                    for j in range(len(a)):
+                        if ds[j] is None:
+                            continue
+                        others = [a[m] for m in range(len(a)) if m != j]
+                        prod = others[0] if len(others) == 1 else self.op("MUL" if len(others) == 2 else "MUL3", *others)
+                        r = add(r, mul(prod, ds[j]))
+                elif name in ("MULADD", "MULSUB", "SUBMUL"):
+                    s = -1 if name == "SUBMUL" else 1
+                    t = add(mul(a[1], ds[0]), mul(a[0], ds[1]))
+                    r = add(t if s > 0 else neg(t), ds[2] if name != "MULSUB" else neg(ds[2]))
+                elif name in ("MULMULADD", "MULMULSUB"):
+                    t1 = add(mul(a[1], ds[0]), mul(a[0], ds[1]))
+                    t2 = add(mul(a[3], ds[2]), mul(a[2], ds[3]))
+                    r = add(t1, t2 if name == "MULMULADD" else neg(t2))
+                elif name == "SQR":
+                    r = mul(self.op("ADD", a[0], a[0]), ds[0])
+                elif name == "CB":
+                    r = mul(self.op("MUL", self.c(3.0), self.op("SQR", a[0])), ds[0])
+                elif name in ("INV", "INV_NOT_ZERO"):
+                    r = mul(self.op("NEG", self.op("INVSQR", a[0])), ds[0])
+                elif name == "INVSQR":
+                    r = mul(self.op("MUL", self.c(-2.0), self.op("POW_INT", a[0], -3)), ds[0])
+                elif name == "DIV":
+                    r = add(self.op("DIV", ds[0], a[1]) if ds[0] is not None else None,
+                            mul(self.op("NEG", self.op("DIV", out, a[1])), ds[1]))
+                elif name == "POW_INT":
+                    k = args[1]
+                    r = mul(self.op("MUL", self.c(float(k)), self.op("POW_INT", a[0], k - 1)), ds[0])
+                else:
+                    raise ValueError(name)
+                if r is not None:
+                    d[(out, v)] = r
+        jac = [[D(o, v) for v in range(self.nvars)] for o in outputs]
+        return jac
+
+    def _slot(self, ref):
+        nc, npar = len(self.consts), self.nparams
+        if ref[0] == "c":
+            return ref[1] + 1
+        if ref[0] == "p":
+            return nc + ref[1] + 1
+        if ref[0] == "x":
+            return nc + npar + ref[1] + 1
+        return nc + npar + self.nvars + ref[1] + 1
+
+    def tapes(self, outputs, extra_jacobian=()):
+        """(eval_jac, eval) Tapes of the outputs.  extra_jacobian: (i, j, ref-builder) terms
+        added to U[i, j] in the Jacobian tape only."""
+        from hcgpu.tape import Tape
+        neval = len(self.ins)
+        jac = self._jacobian_instructions(outputs)
+        for i, j, build in extra_jacobian:
+            term = build(self)
+            jac[i][j] = term if jac[i][j] is None else self.op("ADD", jac[i][j], term)
+        # constants are numbered first: the slots of every instruction are final only now
+
+        def instr(rows):
+            out = []
+            for k, (name, args) in enumerate(rows):
+                ar = [0, 0, 0, 0]
+                for q, a in enumerate(args):
+                    ar[q] = a if isinstance(a, int) else self._slot(a)
+                out.append((*ar, self.OP[name], self._slot(("i", k))))
+            return out
+
+        n = self.nvars
+        u = [(i + 1, self._slot(o)) for i, o in enumerate(outputs)]
+        U = [(j * n + i + 1, self._slot(jac[i][j])) for i in range(n) for j in range(n) if jac[i][j] is not None]
+        ev = Tape(instr(self.ins[:neval]), self.consts, self.nparams, n, u, [], n)
+        ej = Tape(instr(self.ins), self.consts, self.nparams, n, u, U, n)
+        # the eval tape's tape_space must cover the same slot numbering as the Jacobian tape's
+        return ej, ev
+
+
+FUSED_X = np.array([0.5 + 0.2j, -0.3 + 0.6j, 0.4 - 0.1j])
+
+
+def fused_op_system():
+    """F(x; a) = g(x) - a with n = 3, each g_i written with ModelKit's fused and inverse ops:
+      g0 = ADD4(c0 x0, SQR(x1), MULMULADD(x0, x2, c1, x1), IDENTITY(x2))
+      g1 = SUBMUL(x1, x2, ADD3(MUL3(c2, x0, x1), CB(x2), DIV(x0, x1 + c4)))
+      g2 = MUL4(c3, x0, x1, x2) + INV(x0 + c6) + INVSQR(x1 + c7) + POW_INT(x2, 4)
+           + MULMULSUB(x0, x1, x2, c8) + INV_NOT_ZERO(x2 + c9) + POW_INT(x0 + c10, -2)
+      f0 = SUB(g0, a0), f1 = MULSUB(1, g1, a1), f2 = MULADD(c11, g2, NEG(a2))  (c11 = 1)
+    The Jacobian tape also adds INV_NOT_ZERO(x0 - x0) (an exact zero, so the zero branch) to
+    U[0, 0].  Returns (TapeSystem, g as an mpmath function, the start point x*)."""
+    from hcgpu.homotopies import TapeSystem
+    T = HandTape(3, 3)
+    x0, x1, x2 = T.x(0), T.x(1), T.x(2)
+    C = [T.c(v) for v in (0.6 + 0.3j, -0.4 + 0.8j, 1.2 - 0.5j, 0.9 + 0.1j, 2.1 + 0.3j, 0, 1.7 - 0.4j,
+                          -1.9 + 0.2j, 0.35 - 0.7j, 1.4 + 1.1j, -2.2 + 0.5j, 1.0)]
+    g0 = T.op("ADD4", T.op("MUL", C[0], x0), T.op("SQR", x1), T.op("MULMULADD", x0, x2, C[1], x1), T.op("IDENTITY", x2))
+    g1 = T.op("SUBMUL", x1, x2, T.op("ADD3", T.op("MUL3", C[2], x0, x1), T.op("CB", x2),
+                                     T.op("DIV", x0, T.op("ADD", x1, C[4]))))
+    s = T.op("ADD4", T.op("MUL4", C[3], x0, x1, x2), T.op("INV", T.op("ADD", x0, C[6])),
+             T.op("INVSQR", T.op("ADD", x1, C[7])), T.op("POW_INT", x2, 4))
+    g2 = T.op("ADD3", s, T.op("MULMULSUB", x0, x1, x2, C[8]),
+              T.op("ADD", T.op("INV_NOT_ZERO", T.op("ADD", x2, C[9])), T.op("POW_INT", T.op("ADD", x0, C[10]), -2)))
+    f0 = T.op("SUB", g0, T.p(0))
+    f1 = T.op("MULSUB", C[11], g1, T.p(1))
+    f2 = T.op("MULADD", C[11], g2, T.op("NEG", T.p(2)))
+    zero_branch = (0, 0, lambda H: H.op("INV_NOT_ZERO", H.op("SUB", H.x(0), H.x(0))))
+    ej, ev = T.tapes([f0, f1, f2], extra_jacobian=[zero_branch])
+    c = [complex(v) for v in T.consts[:12]]
+
+    def g(x, mp):
+        a, b, z = x
+        k = [mp.mpc(v) for v in c]
+        return [k[0] * a + b ** 2 + (a * z + k[1] * b) + z,
+                (k[2] * a * b + z ** 3 + a / (b + k[4])) - b * z,
+                k[3] * a * b * z + 1 / (a + k[6]) + 1 / (b + k[7]) ** 2 + z ** 4 + (a * b - z * k[8])
+                + 1 / (z + k[9]) + (a + k[10]) ** -2]
+
+    return TapeSystem(ej, ev), g, FUSED_X.copy()

@@ -71,3 +71,74 @@ def test_polyhedral_solve_oracle(name, nz, seed, nstart, nsuccess):
     assert ok.sum() == nsuccess
     assert len(_new_points(np.zeros((0, F.nvars), complex), X[ok], 1e-8)) == nsuccess
     assert max(abs(v) for x in X[ok] for v in F.evaluate(x)) < 1e-9
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_cyclic5_toric_tapes_build_for_every_seed(seed):
+    """Every toric homotopy of every lifting the seed draws lowers to a tape (high powers of s
+    are one OP_POW_INT, not a chain of products) and the cells sum to the mixed volume 70."""
+    from hcgpu.homotopies import ParameterHomotopy
+    from hcgpu.tape import OP
+    P = polyhedral(systems.cyclic(5), only_non_zero=True, seed=seed)
+    assert P.mixed_volume == 70
+    for c in P.cells:
+        H = ParameterHomotopy(P.toric_system(c), [0.0], [1.0])
+        H.desc()
+        ev = H.F.eval
+        assert ev.ninstr < 400
+        top = int(max(e.max() for e in c.s_exponents))
+        if top >= 5:
+            assert OP["POW_INT"] in set(ev.instr[:, 4].tolist())
+    assert P.max_power == max(int(max(e.max() for e in c.s_exponents)) for c in P.cells)
+
+
+def test_lifting_keeps_the_lowest_largest_power():
+    """Of the liftings drawn, the one with the smallest largest power of s is kept."""
+    from hcgpu.polyhedral import PolyhedralStart
+    one = PolyhedralStart(systems.cyclic(5), only_non_zero=True, seed=0, liftings=1).max_power
+    many = PolyhedralStart(systems.cyclic(5), only_non_zero=True, seed=0, liftings=8).max_power
+    assert many <= one
+
+
+def test_large_supports_raise_instead_of_returning_no_cells():
+    """5 dense quartics: 7,875^5 pair tuples (beyond int64): a clear error, never 0 cells."""
+    from hcgpu.configs import dense_system
+    from hcgpu.polyhedral import EnumerationTooLarge
+    F = dense_system(5, 4, seed=0)
+    with pytest.raises(EnumerationTooLarge):
+        paths_to_track(F)
+    with pytest.raises(EnumerationTooLarge):
+        polyhedral(F)
+    assert paths_to_track(F, start_system="total_degree") == 1024
+
+
+def test_only_non_zero_given_explicitly_wins_over_only_torus():
+    (a, b, s), _ = variables(3)
+    G = System([a ** 3 * s ** 15 + a * b * s + b ** 3 + s ** 12, a ** 2 * s ** 9 + a * b ** 2 + b * s ** 3,
+                a ** 2 * b * s ** 5 + a * s ** 8 + b ** 2], 3)
+    assert paths_to_track(G, only_non_zero=True, only_torus=False) == 54
+    assert paths_to_track(G, only_non_zero=False, only_torus=True) == 92
+    assert paths_to_track(G, only_torus=True) == 54
+    assert polyhedral(G, only_non_zero=True, only_torus=False, seed=0).mixed_volume == 54
+
+
+def test_failed_toric_paths_become_polyhedral_failed_results():
+    """A toric path that fails is kept (start, last point, s) and reported as a
+    `polyhedral_failed` path result, so successes + failures = mixed volume."""
+    from hcgpu.solve import polyhedral_failed_results
+    P = polyhedral(systems.cyclic(5), only_non_zero=True, seed=3)
+    calls = []
+
+    def tracker(H, X, p, q):
+        codes, sols = oracle_tracker(H, X, p, q)
+        codes = np.asarray(codes).copy()
+        if not calls:
+            codes[0] = 11  # as if the first path of the first cell stopped (step size too small)
+        calls.append(1)
+        return codes, sols, np.full(len(X), 0.75)
+
+    S, failed = P.start_solutions(tracker=tracker)
+    assert failed == 1 and len(S) + failed == P.mixed_volume
+    prs = polyhedral_failed_results(P.failed, len(S) + 1)
+    assert [r.return_code for r in prs] == ["polyhedral_failed"]
+    assert prs[0].t == 0.75 and prs[0].path_number == len(S) + 1 and prs[0].is_failed()
