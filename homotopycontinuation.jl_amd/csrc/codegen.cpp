@@ -838,7 +838,7 @@ bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::st
     if (nsys < 1 || nsys > 8) { *err = "between 1 and 8 systems per program"; return false; }
     std::ostringstream os;
     os << "// generated by hcgpu codegen (multi-system program)\n";
-    os << "#include \"hcg_arith.h\"\n#include \"hcgpu.h\"\n#include \"hcg_kargs.h\"\n#include \"hcg_sched.h\"\n#include \"hcg_mlp_infer.h\"\n";
+    os << "#include \"hcg_arith.h\"\n#include \"hcgpu_types.h\"\n#include \"hcg_kargs.h\"\n#include \"hcg_sched.h\"\n#include \"hcg_mlp_infer.h\"\n";
     for (int s = 0; s < nsys; s++) {
         os << "namespace hcs" << s << " {\n";
         std::string e;

@@ -1,7 +1,7 @@
 // Kernel argument block of hcg_certify_kernel (shared by the host runtime and the
 // hipRTC-compiled device code; plain C types only).
 #pragma once
-#include "hcgpu.h"
+#include "hcgpu_types.h"
 
 typedef struct hcg_cert_kargs {
     int32_t nsol;

@@ -1,7 +1,7 @@
 // Kernel argument block of hcg_track_kernel (shared by the host runtime and the
 // hipRTC-compiled device code; plain C types only).
 #pragma once
-#include "hcgpu.h"
+#include "hcgpu_types.h"
 
 // Waves per SIMD the tracker kernels are compiled for (register budget 512 / this).
 #ifndef HCG_WAVES_PER_EU

@@ -6,7 +6,7 @@
 // wait until at least `refill` of the wave's live lanes are idle (or all of them are),
 // then take their next paths from the work queue together, with one atomic per wave.
 #pragma once
-#include "hcgpu.h"
+#include "hcgpu_types.h"
 
 #define HCG_DEFAULT_REFILL 8
 

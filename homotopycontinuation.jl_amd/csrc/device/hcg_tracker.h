@@ -13,7 +13,7 @@
 //   valuation          src/valuation.jl:38-228    endgame  src/endgame_tracker.jl:260-911
 #pragma once
 #include "hcg_arith.h"
-#include "hcgpu.h"
+#include "hcgpu_types.h"
 #include "hcg_kargs.h"
 #include "hcg_sched.h"
 #include "hcg_mlp_infer.h"

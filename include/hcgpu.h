@@ -24,6 +24,7 @@
 #include <stddef.h>
 #endif
 #include <stdint.h>
+#include "hcgpu_types.h"
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -88,51 +89,6 @@ typedef struct hcg_homotopy_desc {
     double gamma_re, gamma_im;
 } hcg_homotopy_desc;
 
-/* Options: a POD mirror of TrackerOptions + TrackerParameters (src/tracker.jl:45-140),
- * EndgameOptions (src/endgame_tracker.jl:47-72) and WeightedNormOptions
- * (src/norm.jl:36-40), plus GPU-only knobs.  hcg_default_options fills the
- * reference defaults. */
-typedef struct hcg_options {
-    /* TrackerOptions */
-    int32_t max_steps;
-    int32_t extended_precision;
-    double max_step_size;
-    double max_initial_step_size;
-    double min_step_size;
-    double min_rel_step_size;
-    double terminate_cond;
-    /* TrackerParameters */
-    double a, beta_a, beta_omega_p, beta_tau, strict_beta_tau;
-    int32_t min_newton_iters;
-    /* EndgameOptions */
-    int32_t max_endgame_steps;
-    double endgame_start;
-    int32_t max_endgame_extended_steps;
-    int32_t max_winding_number;
-    double min_cond, min_cond_growth, min_coord_growth;
-    int32_t zero_is_at_infinity, at_infinity_check;
-    int32_t only_nonsingular, refine_steps;
-    double singular_min_accuracy;
-    double val_finite_tol, val_at_infinity_tol;
-    double sing_cond, sing_accuracy, scaling_threshold;
-    /* WeightedNormOptions */
-    double scale_min, scale_abs_min, scale_max;
-    /* GPU-only knobs */
-    int32_t learned_step;     /* 1: use the learned step-size proposer (plain rule stays the judge) */
-    int32_t block_size;       /* 0: autotuned */
-    int32_t blocks_per_cu;    /* 0: autotuned */
-    int32_t log_steps;        /* 1: write per-step feature records into hcg_step_log */
-    double learned_max_mult;  /* largest factor a learned proposal may lengthen the plain one by */
-    double learned_min_t;     /* learned proposals only while |t| > learned_min_t */
-    double learned_inf_t;     /* > 0: a path the network flags as heading to infinity (logit above
-                                 learned_inf_logit) starts its endgame once t <= learned_inf_t instead
-                                 of endgame_start; 0 = the flag is not used */
-    double learned_inf_logit;
-    int32_t refill_idle;         /* scheduling (no counterpart in the reference): finished lanes of a
-                                    wave start their next paths together once at least this many of
-                                    the wave's lanes are idle, or all are; 0 = default 8, 1 = at once */
-    int32_t reserved_i;
-} hcg_options;
 
 /* A batch of paths for one program. */
 typedef struct hcg_batch {
@@ -144,37 +100,6 @@ typedef struct hcg_batch {
     const int32_t* group_of_path; /* npaths entries in [0, ngroups) or NULL (group 0) */
     double t1, t0;                /* track from t1 to t0 (EndgameTracker: 1 -> 0) */
 } hcg_batch;
-
-/* Per-path record, PathResult fields (src/path_result.jl:76-98). */
-typedef struct hcg_path_result {
-    int32_t code;              /* EndgameTrackerCode ordinal */
-    int32_t singular;
-    int32_t winding_number;    /* 0 = nothing */
-    int32_t extended_precision;
-    int32_t extended_precision_used;
-    int32_t accepted_steps;
-    int32_t rejected_steps;
-    int32_t steps_eg;
-    int32_t valuation_valid;   /* 1 when t <= endgame_start (valuation reported) */
-    int32_t newton_iters;      /* GPU extras */
-    int32_t learned_proposed;
-    int32_t learned_accepted;
-    double t, accuracy, residual, cond, omega, mu, last_t;
-} hcg_path_result;
-
-/* Optional per-step record log (training data for the learned proposer; modelled on
- * path_info's per-step table, src/path_info.jl:3-84).  One record per tracker step:
- *  0 path index, 1 step index, 2 t, 3 log Δs, 4 accepted, 5 Newton iterations, 6 θ,
- *  7 log(local error · Δs^4), 8 log10 cond(H_x, ẋ), 9 log ω, 10 log μ, 11 log τ,
- *  12 log of the next proposed Δs, 13 flags (1 extended precision, 2 singular endgame,
- *  4 learned proposal used), 14 log10 accuracy, 15 endgame steps. */
-typedef struct hcg_step_log {
-    int32_t capacity;   /* max records */
-    int32_t pad_;
-    float* records;     /* capacity * HCG_STEP_RECORD_FLOATS */
-    int32_t* count;     /* out: records written (may exceed capacity; extra are dropped) */
-} hcg_step_log;
-#define HCG_STEP_RECORD_FLOATS 16
 
 int hcg_ctx_create(int device, hcg_ctx** out);
 void hcg_ctx_destroy(hcg_ctx* ctx);
@@ -245,12 +170,6 @@ int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* ne
 /* Generate and hipRTC-compile (filling the code-object cache); no device needed. */
 int hcg_compile_only(const hcg_homotopy_desc* h, size_t* code_size);
 
-/* Learned step-size proposer: 8 features -> 64 -> 64 (GELU) -> 2 outputs
- * (log step, at-infinity logit).  Weights are BF16 with FP32 accumulation. */
-#define HCG_MLP_IN 8
-#define HCG_MLP_HIDDEN 64
-#define HCG_MLP_OUT 2
-#define HCG_MLP_NPARAMS (HCG_MLP_IN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_OUT + HCG_MLP_OUT)
 /* Load FP32 master weights (HCG_MLP_NPARAMS floats, layout W1[64][8] b1 W2[64][64] b2 W3[2][64] b3)
  * into the context; the tracker rounds them to BF16. */
 int hcg_mlp_load(hcg_ctx* ctx, const float* params);
@@ -275,16 +194,6 @@ int hcg_mlp_train(hcg_ctx* ctx, float* params, int32_t n, const float* x, const 
  * balls (about 106 bits), the rest of the test staying in FP64 intervals.  Candidates that
  * fail both are reported as not certified.  Duplicate detection among certified boxes
  * (DistinctSolutionCertificates, :300-427) is host work on the returned boxes. */
-typedef struct hcg_certificate {
-    int32_t certified;    /* the Krawczyk test holds: exactly one zero in the box x₀, and it lies in x₁ */
-    int32_t real;         /* that zero is real (conj(x₁) in the interior of x₀, F real on real points) */
-    int32_t complex_;     /* some coordinate of x₁ excludes the real axis: that zero is not real */
-    int32_t newton_code;  /* refinement: 0 success, 1 rejected, 2 max_iters, -1 not refined */
-    int32_t newton_iters;
-    int32_t prec;         /* 53: FP64 residual enclosure; 106: the double-double retry */
-    double contraction;   /* 1.41422 · max_i Σ_j mag(M_ij) for M = C·J(x₀) − I; certified needs < 0.7071 */
-    double inflation;     /* largest inflation radius ε_i of x₀ */
-} hcg_certificate;
 
 /* Compile a certifier for the system F (its eval_jac and eval tapes; F.fixed_params fixes
  * the parameters, else they are given per hcg_certify call).  Counterpart of
