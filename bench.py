@@ -30,22 +30,81 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 METRIC = "paths/sec: learned-step GPU vs plain GPU vs host CPU, same solution sets"
 
 
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _profile_entry(workload: str):
+    try:
+        return json.loads((REPO / "profiles" / "traffic.json").read_text())[workload]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def measured_traffic(workload: str):
     """HBM bytes per launch from the committed FETCH_SIZE / WRITE_SIZE passes of this workload,
     each scaled by what the counter reads for a plain copy of known size; None if not profiled."""
-    try:
-        t = json.loads((REPO / "profiles" / "traffic.json").read_text())[workload]
-    except (OSError, KeyError, ValueError):
+    t = _profile_entry(workload)
+    if t is None or "fetch_kib" not in t:
         return None
     return round((t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0)
 
 
-def roofline(flops, kernel_ms, workload, kernel="hcg_track_kernel"):
+def roofline(flops, kernel_ms, workload, kernel="hcg_track_kernel", info=None, grid_waves=None):
+    """The tracker kernel against its algorithmic roof, MI355X's FP64 vector peak (no MFMA is
+    used: the linear algebra is a few percent of a step's flops at these sizes), with what the
+    profiles measured beside it: HBM traffic per launch (the bound the per-lane kernel actually
+    hits) and, when profiled, the FP64 instruction counters and resident waves."""
     achieved = flops / (kernel_ms * 1e-3) / 1e12
-    return {"bound": "mfma", "bound_detail": "FP64 compute roof (MI355X FP64 vector = matrix peak 78.6 TFLOP/s)",
-            "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": measured_traffic(workload),
-            "kernel": kernel, "kernel_ms": round(kernel_ms, 3), "flops_per_launch": flops}
+    traffic = measured_traffic(workload)
+    r = {"bound": "fp64_vector", "bound_detail": "FP64 vector peak of MI355X, 78.6 TFLOP/s (counted FP64 flops "
+                                                  "of the launch / kernel time)",
+         "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+         "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
+         "kernel": kernel, "kernel_ms": round(kernel_ms, 3), "flops_per_launch": flops}
+    prof = _profile_entry(workload) or {}
+    if traffic is not None:
+        gbs = traffic / (kernel_ms * 1e-3) / 1e9
+        r["measured_bound"] = {"what": "HBM bandwidth (FETCH_SIZE + WRITE_SIZE of a profiled launch of this "
+                                       "workload / this launch's kernel time)",
+                               "achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK_GBS,
+                               "frac": round(gbs / HBM_PEAK_GBS, 4), "profile": prof.get("source")}
+    if "fp64_lane_ops" in prof:
+        r["fp64_counters"] = {"measured_fp64_lane_ops": prof["fp64_lane_ops"],
+                              "counted_flops_same_launch": prof.get("counted_flops"),
+                              "note": prof.get("fp64_note")}
+    if info is not None:
+        occ = dict(info)
+        if grid_waves is not None:
+            occ["grid_waves"] = int(grid_waves)
+            occ["resident_waves_per_simd"] = round(min(grid_waves, 4 * info["cus"] * info["waves_per_simd_built"])
+                                                   / (4 * info["cus"]), 3)
+        if "sq_wave_cycles" in prof and "sq_busy_cycles" in prof:
+            occ["measured_mean_waves_per_simd"] = round(prof["sq_wave_cycles"] / prof["sq_busy_cycles"] / 4.0, 3)
+        r["occupancy"] = occ
+    return r
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def native_oracle():
+    """The oracle for the CPU-baseline timing: rebuilt here with -O3 -march=native."""
+    sys.path.insert(0, str(REPO))
+    import oracle  # CPU baseline only (never the measured GPU path)
+    try:
+        build = oracle.use_native_build()
+    except RuntimeError:
+        build = "gcc -O2 -mavx2 (already loaded)"
+    return oracle, build
 
 
 class DeviceBatch:
@@ -264,7 +323,8 @@ def bench_param10(args, out):
                                "1024 starts x 256 targets = 262144 paths, one persistent launch, learned-step tracker",
                    "n": 10, "paths": npaths, "starts": len(S), "targets": len(Q), "params": 16,
                    "affine_parameter_lift": True, "weights": wsrc},
-        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned"),
+        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned", info=prog.kernel_info(),
+                             grid_waves=-(-npaths // 64)),
         "plain_gpu": {"value": round(plain_value, 1), "unit": "paths/s", "kernel_ms": round(rp["kernel_ms"], 3),
                       "steps_per_path": round(rp["steps"] / npaths, 3),
                       "roofline_frac": round(rp["flops"] / (rp["kernel_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)},
@@ -294,6 +354,7 @@ def bench_param10(args, out):
         okc = (o.code == 1) & (~o.singular)
         relc = np.abs(sol_l[:m][okc] - o.solution[okc]).max(axis=1) / np.maximum(1.0, np.abs(o.solution[okc]).max(axis=1))
         out["cpu_baseline"] = {"value": round(m / dt, 1), "unit": "paths/s", "cores": args.cpu_threads, "kind": "port",
+                               "build": args.oracle_build, "host": host_info(),
                                "sample": f"first {args.cpu_targets} of the 256 targets ({m} paths) of the same batch, "
                                          f"oracle (C restatement of the reference tracker, plain rule), {dt:.1f} s"}
         out["gpu_vs_cpu"] = round(value / (m / dt), 2)
@@ -312,14 +373,16 @@ def bench_dense8(args, out):
     ctx = GpuContext(0)
     # 6,561 paths fill 103 waves, far fewer than the GPU's SIMDs: the kernel is built for one
     # wave per SIMD (the largest register budget) instead of the default four
-    batch = DeviceBatch(ctx, ctx.program(H, waves_per_eu=1), S)
+    prog = ctx.program(H, waves_per_eu=1)
+    batch = DeviceBatch(ctx, prog, S)
     r = timed(batch, make_options(), args.steps, args.warmup)
     rec, _ = batch.results()
     npaths = len(S)
     out.update({"value": round(npaths * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
                 "config": {"workload": "dense8: n=8, 8 dense cubics (165 monomials each), 6561 total-degree paths, "
                                        "one persistent launch, plain tracker", "n": 8, "paths": npaths},
-                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths"),
+                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths", info=prog.kernel_info(),
+                                     grid_waves=-(-npaths // 64)),
                 "steps_per_path": round(r["steps"] / npaths, 3), "solution_counts": counts(rec)})
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
@@ -331,7 +394,7 @@ def bench_dense8(args, out):
         o = oracle.track_batch(Hc, S, make_options(), nthreads=args.cpu_threads)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(npaths / dt, 1), "unit": "paths/s", "cores": args.cpu_threads,
-                               "kind": "port", "sample": f"all {npaths} paths, oracle, {dt:.1f} s"}
+                               "kind": "port", "build": args.oracle_build, "host": host_info(), "sample": f"all {npaths} paths, oracle, {dt:.1f} s"}
         out["parity_vs_oracle"] = {"identical_return_codes": bool((o.code == rec["code"]).all())}
 
 
@@ -366,7 +429,8 @@ def bench_mixed(args, out):
                                        "launch, learned-step tracker; value from the kernel time (inputs are copied "
                                        "from the host each launch)", "paths": total,
                            "weights": "committed (hcgpu/weights/mixed_step_mlp.npy)"},
-                "roofline": roofline(lfl, lms, "mixed_learned", kernel="hcg_track_multi_kernel"),
+                "roofline": roofline(lfl, lms, "mixed_learned", kernel="hcg_track_multi_kernel",
+                                     info=prog.kernel_info(), grid_waves=-(-total // 64)),
                 "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
                 "learned": {"speedup_vs_plain": round(pms / lms, 3)},
                 "identical_return_codes_learned_vs_plain": same,
@@ -390,7 +454,7 @@ def bench_mixed(args, out):
             if not (o.code == g.code[:m]).all():
                 out.setdefault("parity_vs_oracle_sample", {})[f"n{H.n}"] = "return codes differ"
         out["cpu_baseline"] = {"value": round(total / cpu_s, 1), "unit": "paths/s", "cores": args.cpu_threads,
-                               "kind": "port", "sample": "oracle, per size, extrapolated to the full batch: "
+                               "kind": "port", "build": args.oracle_build, "host": host_info(), "sample": "oracle, per size, extrapolated to the full batch: "
                                                          + "; ".join(sample)}
         out["gpu_vs_cpu"] = round(out["value"] / (total / cpu_s), 2)
 
@@ -404,7 +468,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
     ap.add_argument("--no-training", action="store_true", help="skip the proposer-training leg (param10)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="threads of the CPU baseline (default: every CPU this process may use)")
     ap.add_argument("--cpu-targets", type=int, default=32, help="targets of the CPU baseline sample (param10)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one plain device-to-device copy of this many MiB (the reference kernel for "
@@ -424,6 +489,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": nbytes, "read_bytes": nbytes, "written_bytes": nbytes}))
         return
+    args.oracle_build = None
+    if not args.no_cpu:
+        _, args.oracle_build = native_oracle()
     out = {"metric": METRIC, "value": None, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}

@@ -679,6 +679,22 @@ extern "C" int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4) {
     return HCG_OK;
 }
 
+// Resource use of a program's kernel, from its code object: [0] VGPRs per lane (as the
+// loader reports them), [1] private segment bytes per lane, [2] static LDS bytes per
+// workgroup, [3] waves per SIMD the kernel was built for, [4] workgroup size, [5] CUs.
+extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
+    if (!p || !out6) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!p->kernel) return fail(HCG_ERR_INVALID_ARGUMENT, "program has no kernel");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    int regs = 0, priv = 0, lds = 0, maxthr = 0;
+    HIPCHK(hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, p->kernel));
+    HIPCHK(hipFuncGetAttribute(&priv, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, p->kernel));
+    HIPCHK(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, p->kernel));
+    HIPCHK(hipFuncGetAttribute(&maxthr, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, p->kernel));
+    out6[0] = regs; out6[1] = priv; out6[2] = lds; out6[3] = p->waves_per_eu; out6[4] = 64; out6[5] = p->ctx->ncu;
+    return HCG_OK;
+}
+
 // ------------------------------ learned proposer ------------------------------
 extern "C" int hcg_mlp_load(hcg_ctx* ctx, const float* params) {
     if (!ctx || !params) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");

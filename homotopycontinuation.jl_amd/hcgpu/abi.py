@@ -204,6 +204,7 @@ def lib():
     L.hcg_generate.argtypes = [P(hcg_homotopy_desc), ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_size_t),
                                P(ctypes.c_long)]
     L.hcg_last_counters.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
+    L.hcg_program_kernel_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int32)]
     L.hcg_certifier_create.argtypes = [ctypes.c_void_p, P(hcg_system), _I, P(ctypes.c_void_p)]
     L.hcg_certifier_compile_only.argtypes = [P(hcg_system), _I, P(ctypes.c_size_t)]
     L.hcg_certify.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _I, P(_D), _I, P(_D), P(_D), P(_D), P(_I), _I, _I,
@@ -224,5 +225,5 @@ HCGPU_SYMBOLS = [
     "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error", "hcg_compile_only", "hcg_generate",
     "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
     "hcg_track_multi", "hcg_program_create_ex", "hcg_compile_ex", "hcg_certifier_create",
-    "hcg_certifier_compile_only", "hcg_certify",
+    "hcg_certifier_compile_only", "hcg_certify", "hcg_program_kernel_info",
 ]

@@ -112,6 +112,9 @@ class Program:
     def source(self) -> str:
         return lib().hcg_program_source(self._h).decode()
 
+    def kernel_info(self):
+        return _kernel_info(self._h)
+
     def close(self):
         if self._h:
             lib().hcg_program_destroy(self._h)
@@ -201,6 +204,15 @@ def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, ta
     return BatchOutput(rec, sol, last, val, log)
 
 
+def _kernel_info(h):
+    """VGPRs and private-segment bytes per lane, LDS bytes, waves per SIMD built for,
+    workgroup size and CUs of a program's tracker kernel (hcg_program_kernel_info)."""
+    out = (ctypes.c_int32 * 6)()
+    check(lib().hcg_program_kernel_info(h, out))
+    return dict(vgprs=int(out[0]), private_bytes_per_lane=int(out[1]), lds_bytes=int(out[2]),
+                waves_per_simd_built=int(out[3]), workgroup=int(out[4]), cus=int(out[5]))
+
+
 class MultiProgram:
     """Several homotopies compiled into one program (hcg_program_create_multi): one launch
     tracks the paths of all of them from a single work queue."""
@@ -213,6 +225,9 @@ class MultiProgram:
         self.ns = [h.n for h in homotopies]
         self.waves_per_eu = waves_per_eu
         self._h = _create(ctx, [h.desc() for h in homotopies], True, waves_per_eu)
+
+    def kernel_info(self):
+        return _kernel_info(self._h)
 
     def close(self):
         if self._h:

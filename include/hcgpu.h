@@ -164,6 +164,11 @@ int hcg_last_kernel_stats(hcg_ctx* ctx, double* kernel_ms, int64_t* total_steps,
 /* Device counters of the last launch: [0] tracker steps, [1] Newton iterations,
  * [2] counted FP64 flops, [3] accepted learned proposals. */
 int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4);
+/* Resource use of a program's kernel from its code object (no counterpart in the reference:
+ * the occupancy evidence of the measurement): out6 = [VGPRs per lane, private segment bytes
+ * per lane, static LDS bytes per workgroup, waves per SIMD the kernel was built for,
+ * workgroup size, compute units of the device]. */
+int hcg_program_kernel_info(const hcg_program* prog, int32_t* out6);
 /* Generated HIP source and static flop counts (eval, eval_jac, eval_dd, taylor1..3) of a
  * homotopy without compiling it; no device needed. */
 int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6);

@@ -29,6 +29,17 @@ def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
 
+def use_native_build() -> str:
+    """Switch this process to the timing build of the same source (-O3 -march=native, made on
+    this host; same arithmetic contract, so identical results).  Must run before lib()."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), "native"], check=True)  # always for this host
+    LIB = HERE / "_build" / "liboracle_native.so"
+    return "gcc -O3 -march=native -ffp-contract=off"
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -139,7 +150,7 @@ def track_batch(H, starts, opts: hcg_options | None = None, nthreads: int = 0, g
     sol = np.zeros((npaths, n), dtype=np.complex128)
     last = np.zeros((npaths, n), dtype=np.complex128)
     val = np.zeros((npaths, n), dtype=np.float64)
-    nthreads = nthreads or os.cpu_count() or 1
+    nthreads = nthreads or len(os.sched_getaffinity(0))
     log = np.zeros((max(log_capacity, 1), 16), dtype=np.float32)
     cnt = ctypes.c_int(0)
     rc = lib().orc_track_batch_log(ctypes.byref(H.desc()), ctypes.byref(b), ctypes.byref(opts), nthreads, res,
@@ -342,7 +353,7 @@ def certify(system, n, candidates, params=None, params_ival=None, params_ball=No
     BP = np.ascontiguousarray(np.asarray(params_ball, dtype=np.float64)) if params_ball is not None else None
     G = np.ascontiguousarray(np.asarray(group_of, dtype=np.int32)) if group_of is not None else None
     keep = (X, P, IP, BP, G)  # noqa: F841  (alive during the call)
-    nthreads = nthreads or min(os.cpu_count() or 1, 16)
+    nthreads = nthreads or min(len(os.sched_getaffinity(0)), 16)
     opt = lambda a: _dp(a) if a is not None else None  # noqa: E731
     rc = lib().orc_certify(ctypes.cast(ctypes.pointer(system), ctypes.c_void_p), n, nsol, _dp(X), ngroups, opt(P),
                            opt(IP), opt(BP), G.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if G is not None else None,
