@@ -28,6 +28,7 @@
 
 #include "hcg_codegen.h"
 #include "hcg_mlp.h"
+#include "hcg_noalloc.h"
 #include "hcgpu.h"
 
 #include "_gen/device_headers.inc"
@@ -419,15 +420,36 @@ static int ctx_scalars(hcg_ctx* ctx) {
     return HCG_OK;
 }
 
+// Where a launch keeps its queue head, counters and options: the context's own buffers
+// (allocated once per context) or the fixed scratch of the allocation-free entry point.
+struct LaunchState {
+    int32_t* next;
+    unsigned long long* counters;
+    hcg_options* opts;
+    hcg::NoallocScratch* fixed;  // non-null: state lives in the fixed scratch
+};
+
+static int ctx_launch_state(hcg_ctx* ctx, LaunchState* st) {
+    int rc = ctx_scalars(ctx);
+    if (rc) return rc;
+    st->next = ctx->d_next;
+    st->counters = ctx->d_counters;
+    st->opts = ctx->d_opts;
+    st->fixed = nullptr;
+    return HCG_OK;
+}
+
 // Launch the persistent tracker kernel over device-resident inputs/outputs.
 static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
                         hcg_path_result* d_res, double* d_sol, double* d_last, double* d_val, float* d_log, int logcap,
-                        int32_t* d_logc) {
-    int rc = ctx_scalars(ctx);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->d_opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(int32_t), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->d_counters, 0, sizeof(unsigned long long) * 4, ctx->stream));
+                        int32_t* d_logc, const LaunchState& st) {
+    if (st.fixed) {
+        if (hcg::noalloc_init(ctx->stream, st.fixed, opts)) return fail(HCG_ERR_HIP, "scratch init kernel launch failed");
+    } else {
+        HIPCHK(hipMemcpyAsync(st.opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemsetAsync(st.next, 0, sizeof(int32_t), ctx->stream));
+        HIPCHK(hipMemsetAsync(st.counters, 0, sizeof(unsigned long long) * 4, ctx->stream));
+    }
     hcg_kargs ka;
     std::memset(&ka, 0, sizeof ka);
     ka.npaths = b->npaths;
@@ -439,13 +461,13 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
     ka.start_params = b->start_params;
     ka.target_params = b->target_params;
     ka.group_of_path = b->group_of_path;
-    ka.next_path = ctx->d_next;
-    ka.opts = ctx->d_opts;
+    ka.next_path = st.next;
+    ka.opts = st.opts;
     ka.results = d_res;
     ka.solutions = d_sol;
     ka.last_points = d_last;
     ka.valuations = d_val;
-    ka.counters = ctx->d_counters;
+    ka.counters = st.counters;
     ka.log_records = d_log;
     ka.log_count = d_logc;
     ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
@@ -458,7 +480,7 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->last_counters, ctx->d_counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(ctx->last_counters, st.counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
                           ctx->stream));
     return HCG_OK;
 }
@@ -512,8 +534,11 @@ static int hcg_track_batch_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg
         HIPCHK(hipMalloc(&d_logc.p, sizeof(int32_t)));
         HIPCHK(hipMemsetAsync(d_logc.p, 0, sizeof(int32_t), ctx->stream));
     }
+    LaunchState st;
+    rc = ctx_launch_state(ctx, &st);
+    if (rc) return rc;
     rc = launch_track(ctx, prog, &db, opts, (hcg_path_result*)d_res.p, (double*)d_sol.p, (double*)d_last.p,
-                      (double*)d_val.p, (float*)d_log.p, logcap, (int32_t*)d_logc.p);
+                      (double*)d_val.p, (float*)d_log.p, logcap, (int32_t*)d_logc.p, st);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(results, d_res.p, sizeof(hcg_path_result) * np, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(solutions, d_sol.p, sz_starts, hipMemcpyDeviceToHost, ctx->stream));
@@ -541,8 +566,36 @@ static int hcg_track_batch_device_impl(hcg_ctx* ctx, const hcg_program* prog, co
         return fail(HCG_ERR_INVALID_ARGUMENT, "NULL output");
     if (batch_dev->npaths == 0) return HCG_OK;
     HIPCHK(hipSetDevice(ctx->device));
+    LaunchState st;
+    rc = ctx_launch_state(ctx, &st);
+    if (rc) return rc;
     rc = launch_track(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, nullptr,
-                      0, nullptr);
+                      0, nullptr, st);
+    if (rc) return rc;
+    return finish_launch(ctx);
+}
+
+// hcg_track_batch_device without any allocation: the launch state lives in the library's
+// fixed device scratch (noalloc.hip), cleared by a kernel on the context's stream.
+static int hcg_track_batch_device_noalloc_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                               const hcg_options* opts, hcg_path_result* results_dev,
+                                               double* solutions_dev, double* last_points_dev, double* valuations_dev) {
+    int rc = validate_batch_args(ctx, prog, batch_dev, opts, false);
+    if (rc) return rc;
+    if (!results_dev || !solutions_dev || !last_points_dev || !valuations_dev)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL output");
+    if (batch_dev->npaths == 0) return HCG_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hcg::NoallocScratch* fixed = nullptr;
+    const char* err = "";
+    if (hcg::noalloc_scratch(ctx->device, &fixed, &err)) return fail(HCG_ERR_HIP, "hipGetSymbolAddress(scratch): %s", err);
+    LaunchState st;
+    st.next = &fixed->next_path;
+    st.counters = fixed->counters;
+    st.opts = &fixed->opts;
+    st.fixed = fixed;
+    rc = launch_track(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, nullptr,
+                      0, nullptr, st);
     if (rc) return rc;
     return finish_launch(ctx);
 }
@@ -769,6 +822,18 @@ extern "C" int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, con
         return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device: %s", ex.what());
     } catch (...) {
         return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device: unexpected error");
+    }
+}
+extern "C" int hcg_track_batch_device_noalloc(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                              const hcg_options* opts, hcg_path_result* results_dev,
+                                              double* solutions_dev, double* last_points_dev, double* valuations_dev) {
+    try {
+        return hcg_track_batch_device_noalloc_impl(ctx, prog, batch_dev, opts, results_dev, solutions_dev,
+                                                   last_points_dev, valuations_dev);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device_noalloc: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device_noalloc: unexpected error");
     }
 }
 extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches, const hcg_options* opts, hcg_path_result* results, double* solutions, double* last_points, double* valuations) {

@@ -225,5 +225,5 @@ HCGPU_SYMBOLS = [
     "hcg_mlp_forward_backward", "hcg_mlp_train", "hcg_last_error", "hcg_compile_only", "hcg_generate",
     "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
     "hcg_track_multi", "hcg_program_create_ex", "hcg_compile_ex", "hcg_certifier_create",
-    "hcg_certifier_compile_only", "hcg_certify", "hcg_program_kernel_info",
+    "hcg_certifier_compile_only", "hcg_certify", "hcg_program_kernel_info", "hcg_track_batch_device_noalloc",
 ]

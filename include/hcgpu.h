@@ -131,6 +131,18 @@ int hcg_track_batch_device(hcg_ctx* ctx, const hcg_program* prog, const hcg_batc
                            const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
                            double* last_points_dev, double* valuations_dev);
 
+/* hcg_track_batch_device that allocates nothing.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The launch state (work-queue head, the
+ * four launch counters, the options record) lives in one fixed 1 KiB device array of the
+ * library, one per GPU, held until the process ends; a kernel on the context's stream clears
+ * it at each call.  All calls on one GPU share that array, so calls that overlap on one GPU
+ * race.  Inputs, outputs and limits are those of hcg_track_batch_device (the array's size
+ * does not depend on the batch). */
+int hcg_track_batch_device_noalloc(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                   const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
+                                   double* last_points_dev, double* valuations_dev);
+
 /* Several homotopies (1..8, any sizes and kinds) compiled into one program: one
  * persistent launch tracks the paths of all of them from a single work queue.  This is the
  * batch form of calling threaded_solve (src/solve.jl:628-709) once per system; there is
