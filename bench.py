@@ -299,7 +299,7 @@ def bench_param10(args, out):
     starts = np.tile(S, (len(Q), 1))
     groups = np.repeat(np.arange(len(Q), dtype=np.int32), len(S))
     ctx = GpuContext(0)
-    prog = ctx.program(H)
+    prog = ctx.program(H, layout=args.layout)
     batch = DeviceBatch(ctx, prog, starts, H.p, H.q, groups)
     npaths = len(starts)
 
@@ -322,9 +322,11 @@ def bench_param10(args, out):
         "config": {"workload": "param10: n=10 sparse quadrics (20 monomials, coefficients affine in 16 params), "
                                "1024 starts x 256 targets = 262144 paths, one persistent launch, learned-step tracker",
                    "n": 10, "paths": npaths, "starts": len(S), "targets": len(Q), "params": 16,
-                   "affine_parameter_lift": True, "weights": wsrc},
-        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned", info=prog.kernel_info(),
-                             grid_waves=-(-npaths // 64)),
+                   "affine_parameter_lift": True, "weights": wsrc,
+                   "layout": "one path per lane" if args.layout == "lane" else "one path per 16-lane group"},
+        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned" + ("" if args.layout == "lane" else "_group"),
+                             kernel="hcg_track_kernel" if args.layout == "lane" else "hcg_group_kernel",
+                             info=prog.kernel_info(), grid_waves=-(-npaths // 64)),
         "plain_gpu": {"value": round(plain_value, 1), "unit": "paths/s", "kernel_ms": round(rp["kernel_ms"], 3),
                       "steps_per_path": round(rp["steps"] / npaths, 3),
                       "roofline_frac": round(rp["flops"] / (rp["kernel_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)},
@@ -373,7 +375,7 @@ def bench_dense8(args, out):
     ctx = GpuContext(0)
     # 6,561 paths fill 103 waves, far fewer than the GPU's SIMDs: the kernel is built for one
     # wave per SIMD (the largest register budget) instead of the default four
-    prog = ctx.program(H, waves_per_eu=1)
+    prog = ctx.program(H, waves_per_eu=1) if args.layout == "lane" else ctx.program(H, layout="group")
     batch = DeviceBatch(ctx, prog, S)
     r = timed(batch, make_options(), args.steps, args.warmup)
     rec, _ = batch.results()
@@ -406,7 +408,7 @@ def bench_mixed(args, out):
     from hcgpu.solve import make_options
     probs = cfg4_problem()
     ctx = GpuContext(0)
-    prog = ctx.program_multi([H for H, _ in probs])
+    prog = ctx.program_multi([H for H, _ in probs], layout=args.layout)
     starts = [S for _, S in probs]
     total = sum(len(S) for S in starts)
     learned.load(ctx, learned.saved_weights("mixed_step_mlp"))
@@ -465,6 +467,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
+    ap.add_argument("--layout", choices=["lane", "group"], default="lane",
+                    help="lane: one path per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
     ap.add_argument("--no-training", action="store_true", help="skip the proposer-training leg (param10)")

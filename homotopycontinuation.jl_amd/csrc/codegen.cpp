@@ -773,7 +773,8 @@ static void gen_system(std::ostringstream& os, const hcg_system& s, const std::s
 
 // Validates one homotopy and writes its HCG_* macros, system functions and static flop
 // counts (everything a tracker body needs) to `os`.
-static bool emit_system(std::ostringstream& os, const hcg_homotopy_desc* h, Flops* fl, std::string* err) {
+static bool emit_system(std::ostringstream& os, const hcg_homotopy_desc* h, Flops* fl, std::string* err,
+                        long* fl6 = nullptr) {
     int n = h->n;
     if (n < 1 || n > HCG_MAX_N) { *err = "system size out of range (1.." + std::to_string(HCG_MAX_N) + ")"; return false; }
     if (h->kind != HCG_STRAIGHT_LINE && h->kind != HCG_PARAMETER) { *err = "unknown homotopy kind"; return false; }
@@ -811,13 +812,26 @@ static bool emit_system(std::ostringstream& os, const hcg_homotopy_desc* h, Flop
         comb_t1 = 8L * n;
         comb_t23 = 24L * n;
     }
-    os << "#define HCG_FL_EVAL " << fl->eval + comb_eval << "\n";
-    os << "#define HCG_FL_EJ " << fl->eval_jac + comb_ej << "\n";
-    os << "#define HCG_FL_DD " << fl->eval_dd + comb_dd << "\n";
-    os << "#define HCG_FL_T1 " << (h->kind == HCG_STRAIGHT_LINE ? fl->eval + comb_t1 : fl->taylor[1]) << "\n";
-    os << "#define HCG_FL_T2 " << fl->taylor[2] + comb_t23 << "\n";
-    os << "#define HCG_FL_T3 " << fl->taylor[3] + comb_t23 << "\n";
+    const long f6[6] = {fl->eval + comb_eval, fl->eval_jac + comb_ej, fl->eval_dd + comb_dd,
+                        h->kind == HCG_STRAIGHT_LINE ? fl->eval + comb_t1 : fl->taylor[1], fl->taylor[2] + comb_t23,
+                        fl->taylor[3] + comb_t23};
+    if (fl6)
+        for (int i = 0; i < 6; i++) fl6[i] = f6[i];
+    os << "#define HCG_FL_EVAL " << f6[0] << "\n";
+    os << "#define HCG_FL_EJ " << f6[1] << "\n";
+    os << "#define HCG_FL_DD " << f6[2] << "\n";
+    os << "#define HCG_FL_T1 " << f6[3] << "\n";
+    os << "#define HCG_FL_T2 " << f6[4] << "\n";
+    os << "#define HCG_FL_T3 " << f6[5] << "\n";
     return true;
+}
+
+// The counted FP64 flops of one homotopy-level evaluation (eval, eval_jac, eval_dd, taylor 1..3),
+// as the generated kernels count them; also validates the homotopy.
+bool homotopy_flops(const hcg_homotopy_desc* h, long* fl6, std::string* err) {
+    std::ostringstream os;
+    Flops fl;
+    return emit_system(os, h, &fl, err, fl6);
 }
 
 bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, std::string* err) {

@@ -314,7 +314,9 @@ template <int K> HD ser<K> s_mulmulsub(const ser<K>& a, const ser<K>& b, const s
  * cbrt: exp(log|x|/3) plus one Newton correction. */
 #define HC_LN2_HI 6.93147180369123816490e-01
 #define HC_LN2_LO 1.90821492927058770002e-10
-__device__ __noinline__ double hc_log(double x) {
+// The *_inl forms are inlined where the caller keeps much state in registers (a call would
+// save and restore it); the plain names are out-of-line calls of the same code.
+HD double hc_log_inl(double x) {
     if (isnan(x) || x < 0) return HCG_NAN;
     if (x == 0) return -HCG_INF;
     if (isinf(x)) return x;
@@ -331,7 +333,7 @@ __device__ __noinline__ double hc_log(double x) {
     double de = (double)e;
     return de * HC_LN2_HI + (de * HC_LN2_LO + lm);
 }
-__device__ __noinline__ double hc_exp(double x) {
+HD double hc_exp_inl(double x) {
     if (isnan(x)) return x;
     if (x > 709.782712893384) return HCG_INF;
     if (x < -745.2) return 0.0;
@@ -347,22 +349,26 @@ __device__ __noinline__ double hc_exp(double x) {
     if (k < -1000) { p = ldexp(p, -1000); k += 1000; }
     return ldexp(p, k);
 }
-__device__ __noinline__ double hc_pow(double x, double y) {
+HD double hc_pow_inl(double x, double y) {
     if (y == 0) return 1.0;
     if (x == 1.0) return 1.0;
     if (isnan(x) || isnan(y)) return HCG_NAN;
     if (x == 0) return y > 0 ? 0.0 : HCG_INF;
     if (isinf(x)) return y > 0 ? HCG_INF : 0.0;
     if (x < 0) return HCG_NAN;
-    return hc_exp(y * hc_log(x));
+    return hc_exp_inl(y * hc_log_inl(x));
 }
-__device__ __noinline__ double hc_cbrt(double x) {
+HD double hc_cbrt_inl(double x) {
     if (x == 0 || isnan(x) || isinf(x)) return x;
     double a = fabs(x);
-    double y = hc_exp(hc_log(a) / 3.0);
+    double y = hc_exp_inl(hc_log_inl(a) / 3.0);
     y = y - (y * y * y - a) / (3.0 * (y * y));
     return x < 0 ? -y : y;
 }
+__device__ __noinline__ double hc_log(double x) { return hc_log_inl(x); }
+__device__ __noinline__ double hc_exp(double x) { return hc_exp_inl(x); }
+__device__ __noinline__ double hc_pow(double x, double y) { return hc_pow_inl(x, y); }
+__device__ __noinline__ double hc_cbrt(double x) { return hc_cbrt_inl(x); }
 
 // Julia's NaN-propagating min/max/clamp and helpers (src/utils.jl:207-212, 408-422)
 HD double jmin(double a, double b) { if (isnan(a) || isnan(b)) return HCG_NAN; return a < b ? a : b; }
@@ -384,6 +390,14 @@ HD double nthroot(double x, int N) {
     if (N == 1) return x;
     if (N == 0) return 1.0;
     return hc_pow(x, 1.0 / N);
+}
+HD double nthroot_inl(double x, int N) {
+    if (N == 4) return sqrt(sqrt(x));
+    if (N == 2) return sqrt(x);
+    if (N == 3) return hc_cbrt_inl(x);
+    if (N == 1) return x;
+    if (N == 0) return 1.0;
+    return hc_pow_inl(x, 1.0 / N);
 }
 HD double ipow(double x, int n) {
     double r = 1.0, b = x; int k = n < 0 ? -n : n;

@@ -1,0 +1,215 @@
+// Interpreter of lane-group slot programs (hcg_group_types.h).  Lane r of the group executes
+// instruction r of every slot; operands come from the group's LDS registers, the constant
+// table, or the homotopy's parameters formed at t.  Three value types: ComplexF64 (evaluate!,
+// evaluate_and_jacobian!), ComplexDF64 (evaluate! with x in double-double) and truncated
+// Taylor series of order K (taylor!), with the operation semantics of
+// src/model_kit/operations.jl:183-248 and src/model_kit/taylor.jl:272-920 written with the
+// same arithmetic as the oracle's tape interpreter (hcg_arith.h).
+#pragma once
+#include "hcg_arith.h"
+#include "hcg_group_types.h"
+
+struct GEval {
+    const hcg_gprog* P;
+    char* lds;           // this group's register file
+    cplx t;              // parameters are formed at t (parameter homotopies)
+    const double* p;     // start parameters (or null)
+    const double* q;     // target parameters of the path's group (or null)
+};
+
+__device__ __forceinline__ int g_int_imm(int32_t ref) { return ((int)((unsigned)ref << 4)) >> 4; }
+
+// The lanes of a wave execute a slot together (LDS operations of one wave complete in order);
+// this marks the slot boundary for the compiler, so no LDS access moves across it.
+__device__ __forceinline__ void g_slot_end() { __builtin_amdgcn_wave_barrier(); }
+
+// ------------------------------- ComplexF64 -------------------------------
+__device__ __forceinline__ cplx g_fetch_c(const GEval& E, int32_t ref) {
+    const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
+    if (kind == HCG_GK_REG) return ((const cplx*)E.lds)[idx];
+    if (kind == HCG_GK_CONST) return mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]);
+    if (kind == HCG_GK_PARAM) return par_lerp(E.t, E.p, E.q, idx);
+    return mkc(0.0, 0.0);
+}
+__device__ __forceinline__ cplx g_op_c(int op, cplx a, cplx b, cplx c, cplx d, int e) {
+    switch (op) {
+    case 1: return c_cb(a);
+    case 7: return c_inv(a);
+    case 8: return c_inv_not_zero(a);
+    case 9: return c_invsqr(a);
+    case 10: return c_neg(a);
+    case 13: return c_sqr(a);
+    case 17: return a;
+    case 18: return c_add(a, b);
+    case 19: return c_div(a, b);
+    case 20: return c_mul(a, b);
+    case 21: return c_sub(a, b);
+    case 22: return c_pow_int(a, e);
+    case 24: return c_add3(a, b, c);
+    case 25: return c_mul3(a, b, c);
+    case 26: return c_muladd(a, b, c);
+    case 27: return c_mulsub(a, b, c);
+    case 28: return c_submul(a, b, c);
+    case 29: return c_add4(a, b, c, d);
+    case 30: return c_mul4(a, b, c, d);
+    case 31: return c_mulmuladd(a, b, c, d);
+    case 32: return c_mulmulsub(a, b, c, d);
+    default: return mkc(HCG_NAN, HCG_NAN);
+    }
+}
+// Loads lane r's variable value into each tape's variable register, runs every slot.
+__device__ __forceinline__ void g_run_c(GEval E, int r, int n, cplx x) {
+    const hcg_gprog* P = E.P;
+    cplx* R = (cplx*)E.lds;
+    if (r < n)
+        for (int t = 0; t < P->ntapes; t++) {
+            const int vr = P->var_reg[t][r];
+            if (vr >= 0) R[vr] = x;
+        }
+    g_slot_end();
+    const int4* A = (const int4*)P->arg;
+    for (int s = 0; s < P->nslots; s++) {
+        const int op = P->op[s];
+        const int4 a = A[s * HCG_GROUP_LANES + r];
+        const int o = P->out[s * HCG_GROUP_LANES + r];
+        if (o >= 0) {
+            cplx va = g_fetch_c(E, a.x);
+            cplx vb = op == 22 ? mkc(0.0, 0.0) : g_fetch_c(E, a.y);
+            cplx vc = g_fetch_c(E, a.z), vd = g_fetch_c(E, a.w);
+            R[o] = g_op_c(op, va, vb, vc, vd, g_int_imm(a.y));
+        }
+        g_slot_end();
+    }
+}
+
+// ------------------------------- ComplexDF64 -------------------------------
+__device__ __forceinline__ cdd g_fetch_dd(const GEval& E, int32_t ref) {
+    const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
+    if (kind == HCG_GK_REG) return ((const cdd*)E.lds)[idx];
+    if (kind == HCG_GK_CONST) return cdd_from(mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]));
+    if (kind == HCG_GK_PARAM) return cdd_from(par_lerp(E.t, E.p, E.q, idx));
+    return mkcdd(mkdd(0.0, 0.0), mkdd(0.0, 0.0));
+}
+__device__ __forceinline__ cdd g_op_dd(int op, cdd a, cdd b, cdd c, cdd d, int e) {
+    switch (op) {
+    case 1: return cdd_cb(a);
+    case 7: return cdd_inv(a);
+    case 8: return cdd_inv_not_zero(a);
+    case 9: return cdd_invsqr(a);
+    case 10: return cdd_neg(a);
+    case 13: return cdd_sqr(a);
+    case 17: return a;
+    case 18: return cdd_add(a, b);
+    case 19: return cdd_div(a, b);
+    case 20: return cdd_mul(a, b);
+    case 21: return cdd_sub(a, b);
+    case 22: return cdd_pow_int(a, e);
+    case 24: return cdd_add3(a, b, c);
+    case 25: return cdd_mul3(a, b, c);
+    case 26: return cdd_muladd(a, b, c);
+    case 27: return cdd_mulsub(a, b, c);
+    case 28: return cdd_submul(a, b, c);
+    case 29: return cdd_add4(a, b, c, d);
+    case 30: return cdd_mul4(a, b, c, d);
+    case 31: return cdd_mulmuladd(a, b, c, d);
+    case 32: return cdd_mulmulsub(a, b, c, d);
+    default: return mkcdd(mkdd(HCG_NAN, 0.0), mkdd(HCG_NAN, 0.0));
+    }
+}
+__device__ __forceinline__ void g_run_dd(GEval E, int r, int n, cplx x) {
+    const hcg_gprog* P = E.P;
+    cdd* R = (cdd*)E.lds;
+    if (r < n)
+        for (int t = 0; t < P->ntapes; t++) {
+            const int vr = P->var_reg[t][r];
+            if (vr >= 0) R[vr] = cdd_from(x);
+        }
+    g_slot_end();
+    const int4* A = (const int4*)P->arg;
+    for (int s = 0; s < P->nslots; s++) {
+        const int op = P->op[s];
+        const int4 a = A[s * HCG_GROUP_LANES + r];
+        const int o = P->out[s * HCG_GROUP_LANES + r];
+        if (o >= 0) {
+            cdd va = g_fetch_dd(E, a.x);
+            cdd vb = op == 22 ? cdd_from(mkc(0.0, 0.0)) : g_fetch_dd(E, a.y);
+            cdd vc = g_fetch_dd(E, a.z), vd = g_fetch_dd(E, a.w);
+            R[o] = g_op_dd(op, va, vb, vc, vd, g_int_imm(a.y));
+        }
+        g_slot_end();
+    }
+}
+
+// --------------------------- Taylor series, order K ---------------------------
+template <int K> __device__ __forceinline__ ser<K> g_fetch_s(const GEval& E, int32_t ref) {
+    const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
+    if (kind == HCG_GK_REG) return ((const ser<K>*)E.lds)[idx];
+    if (kind == HCG_GK_CONST) return s_const<K>(mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]));
+    if (kind == HCG_GK_PARAM) {  // p(t + s) = p(t) + s (p - q)
+        ser<K> r = s_const<K>(par_lerp(E.t, E.p, E.q, idx));
+        r.c[1] = par_diff(E.p, E.q, idx);
+        return r;
+    }
+    return s_const<K>(mkc(0.0, 0.0));
+}
+template <int K> __device__ __forceinline__ ser<K> g_op_s(int op, const ser<K>& a, const ser<K>& b, const ser<K>& c,
+                                                          const ser<K>& d, int e) {
+    switch (op) {
+    case 1: return s_cb<K>(a);
+    case 7: return s_inv<K>(a);
+    case 8: return s_inv<K>(a);
+    case 9: return s_invsqr<K>(a);
+    case 10: return s_neg<K>(a);
+    case 13: return s_sqr<K>(a);
+    case 17: return a;
+    case 18: return s_add<K>(a, b);
+    case 19: return s_div<K>(a, b);
+    case 20: return s_mul<K>(a, b);
+    case 21: return s_sub<K>(a, b);
+    case 22: return s_pow_int<K>(a, e);
+    case 24: return s_add3<K>(a, b, c);
+    case 25: return s_mul3<K>(a, b, c);
+    case 26: return s_muladd<K>(a, b, c);
+    case 27: return s_mulsub<K>(a, b, c);
+    case 28: return s_submul<K>(a, b, c);
+    case 29: return s_add4<K>(a, b, c, d);
+    case 30: return s_mul4<K>(a, b, c, d);
+    case 31: return s_mulmuladd<K>(a, b, c, d);
+    case 32: return s_mulmulsub<K>(a, b, c, d);
+    default: return s_const<K>(mkc(HCG_NAN, HCG_NAN));
+    }
+}
+// x0..x2: lane r's entries of the Taylor vectors of orders 0..K-1 (order K of x is zero)
+template <int K> __device__ __forceinline__ void g_run_s(GEval E, int r, int n, cplx x0, cplx x1, cplx x2) {
+    const hcg_gprog* P = E.P;
+    ser<K>* R = (ser<K>*)E.lds;
+    if (r < n) {
+        ser<K> xs = s_const<K>(x0);
+        if (K >= 2) xs.c[1] = x1;
+        if (K >= 3) xs.c[2] = x2;
+        for (int t = 0; t < P->ntapes; t++) {
+            const int vr = P->var_reg[t][r];
+            if (vr >= 0) R[vr] = xs;
+        }
+    }
+    g_slot_end();
+    const int4* A = (const int4*)P->arg;
+    for (int s = 0; s < P->nslots; s++) {
+        const int op = P->op[s];
+        const int4 a = A[s * HCG_GROUP_LANES + r];
+        const int o = P->out[s * HCG_GROUP_LANES + r];
+        if (o >= 0) {
+            ser<K> va = g_fetch_s<K>(E, a.x);
+            ser<K> vb = op == 22 ? s_const<K>(mkc(0.0, 0.0)) : g_fetch_s<K>(E, a.y);
+            ser<K> vc = g_fetch_s<K>(E, a.z), vd = g_fetch_s<K>(E, a.w);
+            R[o] = g_op_s<K>(op, va, vb, vc, vd, g_int_imm(a.y));
+        }
+        g_slot_end();
+    }
+}
+// output reads after a run
+__device__ __forceinline__ cplx g_read_c(const GEval& E, int32_t ref) { return g_fetch_c(E, ref); }
+__device__ __forceinline__ cdd g_read_dd(const GEval& E, int32_t ref) { return g_fetch_dd(E, ref); }
+template <int K> __device__ __forceinline__ cplx g_read_s(const GEval& E, int32_t ref, int k) {
+    return g_fetch_s<K>(E, ref).c[k];
+}

@@ -19,6 +19,7 @@ bool generate_source(const hcg_homotopy_desc* h, std::string* src, Flops* fl, st
 // Certifier for a square system: point, double-double and interval evaluations plus the
 // Krawczyk kernel (device/hcg_certify.h).
 bool generate_certify_source(const hcg_system* F, int n, std::string* src, std::string* err);
+bool homotopy_flops(const hcg_homotopy_desc* h, long* fl6, std::string* err);
 bool generate_multi_source(const hcg_homotopy_desc* const* hs, int nsys, std::string* src, Flops* fls,
                            std::string* err);
 

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "hcg_codegen.h"
+#include "hcg_group_launch.h"
 #include "hcg_mlp.h"
 #include "hcg_noalloc.h"
 #include "hcgpu.h"
@@ -79,6 +80,8 @@ struct hcg_program {
     // certifier programs (hcg_certifier_create): the kernel is hcg_certify_kernel
     bool certifier = false;
     int cert_npar = 0;  // parameters given per hcg_certify call (0 when fixed or absent)
+    // lane-group layout (hcg_build_options.layout = 1): slot programs for the precompiled kernel
+    hcg::GroupPrograms* group = nullptr;
 };
 
 extern "C" const char* hcg_last_error(void) { return g_err.c_str(); }
@@ -315,10 +318,44 @@ static int hcg_compile_ex_impl(const hcg_homotopy_desc* const* hs, int32_t nsys,
     return HCG_OK;
 }
 
+static int group_program_create(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
+                                hcg_program** out) {
+    if (!hs || nsys < 1 || nsys > HCG_MAX_SYSTEMS) return fail(HCG_ERR_INVALID_ARGUMENT, "between 1 and 8 homotopies per program");
+    if (nsys > 1 && !multi) return fail(HCG_ERR_INVALID_ARGUMENT, "several homotopies need a multi-system program");
+    hcg_program* p = new hcg_program();
+    p->ctx = ctx;
+    p->group = new hcg::GroupPrograms();
+    std::string err;
+    if (!hcg::group_build(hs, nsys, true, p->group, &err)) {
+        delete p->group;
+        delete p;
+        return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+    }
+    p->waves_per_eu = 1;
+    p->source = "// lane-group layout: slot programs for the precompiled hcg_group_kernel\n";
+    if (multi) {
+        p->nsys = nsys;
+        for (int s = 0; s < nsys; s++) {
+            p->ns.push_back(hs[s]->n);
+            p->kinds.push_back(hs[s]->kind);
+            p->npars.push_back(hs[s]->kind == HCG_PARAMETER ? hs[s]->target.eval->nparams : 0);
+        }
+        p->n = *std::max_element(p->ns.begin(), p->ns.end());
+    } else {
+        p->n = hs[0]->n;
+        p->kind = hs[0]->kind;
+        p->npar = hs[0]->kind == HCG_PARAMETER ? hs[0]->target.eval->nparams : 0;
+    }
+    *out = p;
+    return HCG_OK;
+}
+
 static int hcg_program_create_ex_impl(hcg_ctx* ctx, const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t multi,
                                      const hcg_build_options* bo, hcg_program** out) {
     if (!ctx || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     HIPCHK(hipSetDevice(ctx->device));
+    if (bo && bo->layout == HCG_LAYOUT_GROUP) return group_program_create(ctx, hs, nsys, multi, out);
+    if (bo && bo->layout != HCG_LAYOUT_LANE) return fail(HCG_ERR_INVALID_ARGUMENT, "unknown layout %d", bo->layout);
     hcg_program* p = new hcg_program();
     p->ctx = ctx;
     std::vector<hcg::Flops> fl;
@@ -368,6 +405,12 @@ extern "C" int hcg_program_create_multi(hcg_ctx* ctx, const hcg_homotopy_desc* c
 
 extern "C" void hcg_program_destroy(hcg_program* p) {
     if (!p) return;
+    if (p->group) {
+        hipSetDevice(p->ctx->device);
+        hcg::group_free(p->group);
+        delete p->group;
+        p->group = nullptr;
+    }
     if (p->module) { hipSetDevice(p->ctx->device); hipModuleUnload(p->module); }
     delete p;
 }
@@ -417,6 +460,20 @@ static int ctx_scalars(hcg_ctx* ctx) {
     HIPCHK(hipMalloc(&ctx->d_next, sizeof(int32_t)));
     HIPCHK(hipMalloc(&ctx->d_counters, sizeof(unsigned long long) * 4));
     HIPCHK(hipMalloc(&ctx->d_opts, sizeof(hcg_options)));
+    return HCG_OK;
+}
+
+// Launch the lane-group kernel: 16 paths per workgroup of 256 lanes, at most
+// blocks_per_cu workgroups per CU (default 2).
+static int group_launch(hcg_ctx* ctx, const hcg_program* prog, const hcg_multi_kargs& m, long total,
+                        const hcg_options* opts) {
+    const int per_block = hcg::group_block() / HCG_GROUP_LANES;
+    const int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 2;
+    long grid = std::min<long>((total + per_block - 1) / per_block, (long)ctx->ncu * per_cu);
+    if (grid < 1) grid = 1;
+    hipError_t e = hcg::group_launch(prog->group->nn, (int)grid, prog->group->lds_per_path, ctx->stream, m,
+                                     prog->group->d_sys);
+    if (e != hipSuccess) return fail(HCG_ERR_HIP, "group kernel launch: %s", hipGetErrorString(e));
     return HCG_OK;
 }
 
@@ -471,14 +528,28 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
     ka.log_records = d_log;
     ka.log_count = d_logc;
     ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
-    const int block = 64;
-    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
-    long waves_needed = ((long)b->npaths + block - 1) / block;
-    long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
-    if (grid < 1) grid = 1;
-    void* args[] = {&ka};
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    if (prog->group) {
+        hcg_multi_kargs m;
+        std::memset(&m, 0, sizeof m);
+        m.nsys = 1;
+        m.total = b->npaths;
+        m.offset[0] = 0;
+        m.offset[1] = b->npaths;
+        m.next_path = st.next;
+        m.sys[0] = ka;
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        int rc = group_launch(ctx, prog, m, b->npaths, opts);
+        if (rc) return rc;
+    } else {
+        const int block = 64;
+        int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
+        long waves_needed = ((long)b->npaths + block - 1) / block;
+        long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
+        if (grid < 1) grid = 1;
+        void* args[] = {&ka};
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->last_counters, st.counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -706,7 +777,13 @@ static int hcg_track_multi_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg
     if (grid < 1) grid = 1;
     void* args[] = {&m};
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    if (prog->group) {
+        for (int s = 0; s < S; s++) m.sys[s].counters = ctx->d_counters, m.sys[s].opts = ctx->d_opts;
+        rc = group_launch(ctx, prog, m, total, opts);
+        if (rc) return rc;
+    } else {
+        HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->last_counters, ctx->d_counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -732,11 +809,45 @@ extern "C" int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4) {
     return HCG_OK;
 }
 
+// Slot schedules of the lane-group layout without a device (checked against the tapes):
+// out = [kernel size class, LDS bytes per path, then per system: eval_jac slots, registers,
+// instructions, eval slots, registers, instructions].
+extern "C" int hcg_group_schedule_info(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t* out) {
+    if (!hs || !out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    try {
+        hcg::GroupPrograms g;
+        std::string err;
+        if (!hcg::group_build(hs, nsys, false, &g, &err)) return fail(HCG_ERR_INVALID_ARGUMENT, "%s", err.c_str());
+        out[0] = g.nn;
+        out[1] = g.lds_per_path;
+        for (int s = 0; s < nsys; s++) {
+            int32_t* o = out + 2 + 6 * s;
+            o[0] = g.ej_slots[s]; o[1] = g.ej_regs[s]; o[2] = g.ej_instr[s];
+            o[3] = g.e_slots[s]; o[4] = g.e_regs[s]; o[5] = g.e_instr[s];
+        }
+        return HCG_OK;
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_group_schedule_info: %s", ex.what());
+    }
+}
+
 // Resource use of a program's kernel, from its code object: [0] VGPRs per lane (as the
 // loader reports them), [1] private segment bytes per lane, [2] static LDS bytes per
 // workgroup, [3] waves per SIMD the kernel was built for, [4] workgroup size, [5] CUs.
 extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
     if (!p || !out6) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (p->group) {
+        HIPCHK(hipSetDevice(p->ctx->device));
+        hipFuncAttributes fa;
+        HIPCHK(hcg::group_kernel_attributes(p->group->nn, &fa));
+        out6[0] = fa.numRegs;
+        out6[1] = (int32_t)fa.localSizeBytes;
+        out6[2] = hcg::group_mlp_lds_bytes() + (hcg::group_block() / HCG_GROUP_LANES) * p->group->lds_per_path;
+        out6[3] = 1;
+        out6[4] = hcg::group_block();
+        out6[5] = p->ctx->ncu;
+        return HCG_OK;
+    }
     if (!p->kernel) return fail(HCG_ERR_INVALID_ARGUMENT, "program has no kernel");
     HIPCHK(hipSetDevice(p->ctx->device));
     int regs = 0, priv = 0, lds = 0, maxthr = 0;

@@ -127,7 +127,7 @@ def default_options(**overrides) -> hcg_options:
 
 
 class hcg_build_options(ctypes.Structure):
-    _fields_ = [("waves_per_eu", _I), ("reserved", _I * 7)]
+    _fields_ = [("waves_per_eu", _I), ("layout", _I), ("reserved", _I * 6)]
 
 
 class hcg_batch(ctypes.Structure):
@@ -205,6 +205,7 @@ def lib():
                                P(ctypes.c_long)]
     L.hcg_last_counters.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
     L.hcg_program_kernel_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int32)]
+    L.hcg_group_schedule_info.argtypes = [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_int32)]
     L.hcg_certifier_create.argtypes = [ctypes.c_void_p, P(hcg_system), _I, P(ctypes.c_void_p)]
     L.hcg_certifier_compile_only.argtypes = [P(hcg_system), _I, P(ctypes.c_size_t)]
     L.hcg_certify.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _I, P(_D), _I, P(_D), P(_D), P(_D), P(_I), _I, _I,
@@ -226,4 +227,5 @@ HCGPU_SYMBOLS = [
     "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
     "hcg_track_multi", "hcg_program_create_ex", "hcg_compile_ex", "hcg_certifier_create",
     "hcg_certifier_compile_only", "hcg_certify", "hcg_program_kernel_info", "hcg_track_batch_device_noalloc",
+    "hcg_group_schedule_info",
 ]

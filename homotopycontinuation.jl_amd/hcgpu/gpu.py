@@ -24,19 +24,22 @@ class GpuContext:
     def handle(self):
         return self._h
 
-    def program(self, homotopy, waves_per_eu: int = 0) -> "Program":
-        key = (id(homotopy), waves_per_eu)
+    def program(self, homotopy, waves_per_eu: int = 0, layout: str = "lane") -> "Program":
+        """layout "lane": one path per lane, the tapes compiled with hipRTC; "group": one path
+        per group of 16 lanes on the precompiled lane-group kernel (systems of up to 16
+        variables)."""
+        key = (id(homotopy), waves_per_eu, layout)
         p = self._programs.get(key)
         if p is None:
-            p = Program(self, homotopy, waves_per_eu)
+            p = Program(self, homotopy, waves_per_eu, layout)
             self._programs[key] = p
         return p
 
-    def program_multi(self, homotopies, waves_per_eu: int = 0) -> "MultiProgram":
-        key = (tuple(id(h) for h in homotopies), waves_per_eu)
+    def program_multi(self, homotopies, waves_per_eu: int = 0, layout: str = "lane") -> "MultiProgram":
+        key = (tuple(id(h) for h in homotopies), waves_per_eu, layout)
         p = self._programs.get(key)
         if p is None:
-            p = MultiProgram(self, list(homotopies), waves_per_eu)
+            p = MultiProgram(self, list(homotopies), waves_per_eu, layout)
             self._programs[key] = p
         return p
 
@@ -85,11 +88,32 @@ def default_context() -> GpuContext:
         return _default_ctx
 
 
-def _create(ctx, descs, multi: bool, waves_per_eu: int):
+LAYOUTS = {"lane": 0, "group": 1}
+
+
+def group_schedule_info(homotopies):
+    """Slot schedules of the lane-group layout for these homotopies (no device needed): the
+    kernel's size class, LDS bytes per path and, per homotopy, slots / registers /
+    instructions of its eval_jac and eval programs."""
+    descs = [h.desc() for h in homotopies]
+    arr = (ctypes.POINTER(abi.hcg_homotopy_desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
+    out = (ctypes.c_int32 * (2 + 6 * len(descs)))()
+    check(lib().hcg_group_schedule_info(ctypes.cast(arr, ctypes.c_void_p), len(descs), out))
+    systems = []
+    for s in range(len(descs)):
+        o = out[2 + 6 * s: 8 + 6 * s]
+        systems.append(dict(ej_slots=o[0], ej_regs=o[1], ej_instr=o[2], e_slots=o[3], e_regs=o[4], e_instr=o[5]))
+    return dict(nn=out[0], lds_per_path=out[1], systems=systems)
+
+
+def _create(ctx, descs, multi: bool, waves_per_eu: int, layout: str = "lane"):
+    if layout not in LAYOUTS:
+        raise ValueError(f"layout must be one of {sorted(LAYOUTS)}")
     h = ctypes.c_void_p()
     arr = (ctypes.POINTER(abi.hcg_homotopy_desc) * len(descs))(*[ctypes.pointer(d) for d in descs])
     bo = abi.hcg_build_options()
     bo.waves_per_eu = waves_per_eu
+    bo.layout = LAYOUTS[layout]
     L = lib()
     L.hcg_program_create_ex.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.POINTER(abi.hcg_homotopy_desc)),
                                         ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(abi.hcg_build_options),
@@ -102,12 +126,13 @@ class Program:
     """A homotopy lowered to HIP and compiled with hipRTC (hcg_program_create_ex);
     waves_per_eu = 0 takes the library's default register budget."""
 
-    def __init__(self, ctx: GpuContext, homotopy, waves_per_eu: int = 0):
+    def __init__(self, ctx: GpuContext, homotopy, waves_per_eu: int = 0, layout: str = "lane"):
         self.ctx = ctx
         self.homotopy = homotopy
         self.n = homotopy.n
         self.waves_per_eu = waves_per_eu
-        self._h = _create(ctx, [homotopy.desc()], False, waves_per_eu)
+        self.layout = layout
+        self._h = _create(ctx, [homotopy.desc()], False, waves_per_eu, layout)
 
     def source(self) -> str:
         return lib().hcg_program_source(self._h).decode()
@@ -217,14 +242,15 @@ class MultiProgram:
     """Several homotopies compiled into one program (hcg_program_create_multi): one launch
     tracks the paths of all of them from a single work queue."""
 
-    def __init__(self, ctx: GpuContext, homotopies, waves_per_eu: int = 0):
+    def __init__(self, ctx: GpuContext, homotopies, waves_per_eu: int = 0, layout: str = "lane"):
         if not 1 <= len(homotopies) <= 8:
             raise ValueError("between 1 and 8 homotopies per program")
         self.ctx = ctx
         self.homotopies = homotopies
         self.ns = [h.n for h in homotopies]
         self.waves_per_eu = waves_per_eu
-        self._h = _create(ctx, [h.desc() for h in homotopies], True, waves_per_eu)
+        self.layout = layout
+        self._h = _create(ctx, [h.desc() for h in homotopies], True, waves_per_eu, layout)
 
     def kernel_info(self):
         return _kernel_info(self._h)

@@ -152,10 +152,17 @@ int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, siz
 
 /* Build choices of a program (no counterpart in the reference; picked by the autotuner).
  * waves_per_eu: waves per SIMD the tracker kernel is compiled for, i.e. a register budget
- * of 512 / waves_per_eu per lane (1, 2, 4 or 8; 0 = the library default). */
+ * of 512 / waves_per_eu per lane (1, 2, 4 or 8; 0 = the library default; per-lane layout).
+ * layout: HCG_LAYOUT_LANE (one path per lane, the tapes compiled with hipRTC) or
+ * HCG_LAYOUT_GROUP (one path per group of 16 lanes, each lane holding one row of the
+ * Jacobian and one entry of every vector in registers; the tapes are scheduled for a
+ * precompiled kernel, so no compilation happens; systems of up to 16 variables). */
+#define HCG_LAYOUT_LANE 0
+#define HCG_LAYOUT_GROUP 1
 typedef struct hcg_build_options {
     int32_t waves_per_eu;
-    int32_t reserved[7];
+    int32_t layout;
+    int32_t reserved[6];
 } hcg_build_options;
 /* General form of hcg_program_create / hcg_program_create_multi (multi = 1: one kernel over
  * all nsys homotopies; multi = 0 requires nsys = 1); bo may be NULL. */
@@ -181,6 +188,12 @@ int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4);
  * per lane, static LDS bytes per workgroup, waves per SIMD the kernel was built for,
  * workgroup size, compute units of the device]. */
 int hcg_program_kernel_info(const hcg_program* prog, int32_t* out6);
+/* The lane-group layout's slot schedules of nsys homotopies, built and checked against their
+ * tapes without a device (no counterpart in the reference; the scheduling evidence of the
+ * layout): out[0] = kernel size class (4, 8, 12, 16), out[1] = LDS bytes per path, then per
+ * system six entries: eval_jac slots, LDS registers, instructions; eval slots, registers,
+ * instructions.  out holds 2 + 6 nsys entries. */
+int hcg_group_schedule_info(const hcg_homotopy_desc* const* hs, int32_t nsys, int32_t* out);
 /* Generated HIP source and static flop counts (eval, eval_jac, eval_dd, taylor1..3) of a
  * homotopy without compiling it; no device needed. */
 int hcg_generate(const hcg_homotopy_desc* h, char* buf, size_t bufsz, size_t* needed, long* flops6);

@@ -1,0 +1,141 @@
+"""The lane-group layout (one path per 16 lanes, hcg_build_options.layout = group) against the
+oracle on the GPU parity cases: identical return codes, singular flags, winding numbers,
+accepted and rejected step counts, and endpoints bit for bit (the layout performs the
+oracle's operations in the oracle's order), plus the mixed-size multi-system launch and the
+learned proposer."""
+import numpy as np
+import pytest
+
+import oracle
+import systems
+from hcgpu.configs import cfg2_problem, cfg4_problem, dense_system
+from hcgpu.gpu import track_batch, track_multi
+from hcgpu.homotopies import ParameterHomotopy, StraightLineHomotopy, total_degree
+from hcgpu.solve import make_options
+
+pytestmark = pytest.mark.gpu
+
+
+def run_group(gpu_ctx, H, S, **kw):
+    opts = make_options(**kw)
+    g = track_batch(gpu_ctx.program(H, layout="group"), S, opts)
+    o = oracle.track_batch(H, S, make_options(**{k: v for k, v in kw.items() if k != "learned_step"}))
+    return g, o
+
+
+def assert_same(g, o, label=""):
+    assert list(g.code) == list(o.code), (label, g.codes()[:8], o.codes()[:8])
+    assert list(g.singular) == list(o.singular), label
+    assert list(g.winding_number) == list(o.winding_number), label
+    assert (g.accepted_steps == o.accepted_steps).all(), label
+    assert (g.rejected_steps == o.rejected_steps).all(), label
+    fin = np.isfinite(o.solution).all(axis=1)
+    np.testing.assert_array_equal(g.solution[fin], o.solution[fin], err_msg=label)
+
+
+def test_group_smoke_case(gpu_ctx):
+    H, S = total_degree(systems.beyond_polyhedral(), gamma=1.3j + 0.4)
+    g, o = run_group(gpu_ctx, H, S)
+    print("group", g.codes(), g.accepted_steps, "oracle", o.codes(), o.accepted_steps)
+    assert_same(g, o)
+
+
+def test_group_parameter_tracker_case(gpu_ctx):
+    H = ParameterHomotopy(systems.tracker_param_system(), [1, 0], [2, 4])
+    g, o = run_group(gpu_ctx, H, np.array([[1, 1]], dtype=complex))
+    assert_same(g, o)
+
+
+@pytest.mark.parametrize("name", ["dense4", "cyclic7", "mohab", "hyperbolic", "winding4", "wilkinson12"])
+def test_group_total_degree_cases(gpu_ctx, name):
+    if name == "dense4":
+        H, S = total_degree(dense_system(4, 3, seed=100), seed=101)
+    elif name == "cyclic7":
+        H, S = total_degree(systems.cyclic(7), seed=4)
+    elif name == "mohab":
+        H, S = total_degree(systems.mohab(), gamma=systems.MOHAB_GAMMA)
+    elif name == "hyperbolic":
+        H, S = total_degree(systems.hyperbolic(), seed=5)
+    elif name == "winding4":
+        H, S = total_degree(systems.winding_family(4), seed=3)
+    else:
+        H, S = total_degree(systems.wilkinson(12), seed=2)
+    g, o = run_group(gpu_ctx, H, S)
+    assert_same(g, o, name)
+
+
+def test_group_parameter_workload_sample(gpu_ctx):
+    F, p0, Q = cfg2_problem(2)
+    from pathlib import Path
+    S = np.load(Path(__file__).resolve().parent / "golden" / "cfg2_starts.npy")
+    H = ParameterHomotopy(F, p0, Q, lift_affine=True)
+    starts = np.tile(S, (2, 1))
+    groups = np.repeat(np.arange(2, dtype=np.int32), len(S))
+    g = track_batch(gpu_ctx.program(H, layout="group"), starts, make_options(), group_of_path=groups)
+    print("group kernel ms", gpu_ctx.last_kernel_ms(), gpu_ctx.last_counters())
+    o = oracle.track_batch(H, starts, make_options(), group_of_path=groups)
+    assert_same(g, o)
+
+
+def test_group_steiner_and_invalid_starts(gpu_ctx):
+    F, p, q, s_p, s_q = systems.steiner()
+    g, o = run_group(gpu_ctx, ParameterHomotopy(F, p, q), s_p[None, :])
+    assert_same(g, o, "steiner")
+    assert g.extended_precision_used[0] == 1
+    G, F2, S = systems.straight_line_case()
+    S = np.vstack([S, [[100, -100]]]).astype(complex)
+    g, o = run_group(gpu_ctx, StraightLineHomotopy(G, F2), S)
+    assert_same(g, o, "straight line")
+    F3, b0, b1, start = systems.framework_454()
+    g, o = run_group(gpu_ctx, ParameterHomotopy(F3, b0, b1), start[None, :])
+    assert_same(g, o, "framework 454")
+    assert g.codes() == ["terminated_invalid_startvalue_singular_jacobian"]
+
+
+def test_group_mixed_sizes_one_launch(gpu_ctx):
+    probs = cfg4_problem()
+    Hs = [H for H, _ in probs]
+    starts = [S if H.n < 12 else S[:2048] for H, S in probs]
+    outs = track_multi(gpu_ctx.program_multi(Hs, layout="group"), starts, make_options())
+    print("group multi launch ms", gpu_ctx.last_kernel_ms(), gpu_ctx.last_counters())
+    for H, S, g in zip(Hs, starts, outs):
+        assert_same(g, oracle.track_batch(H, S, make_options()), f"n={H.n}")
+
+
+def test_group_learned_equals_lane_learned(gpu_ctx):
+    """The learned proposer runs in the group layout (network evaluated by the 16 lanes of a
+    path together, sums in the per-lane order): results equal the per-lane kernel's."""
+    from hcgpu import learned
+    F, p0, Q = cfg2_problem(1)
+    from pathlib import Path
+    S = np.load(Path(__file__).resolve().parent / "golden" / "cfg2_starts.npy")
+    H = ParameterHomotopy(F, p0, Q, lift_affine=True)
+    learned.load(gpu_ctx, learned.saved_weights("param10_step_mlp"))
+    lane = track_batch(gpu_ctx.program(H), S, make_options(learned_step=1))
+    grp = track_batch(gpu_ctx.program(H, layout="group"), S, make_options(learned_step=1))
+    assert list(grp.code) == list(lane.code)
+    assert (grp.accepted_steps == lane.accepted_steps).all()
+    assert (grp.records["learned_accepted"] == lane.records["learned_accepted"]).all()
+    np.testing.assert_array_equal(grp.solution, lane.solution)
+
+
+def test_group_refuses_large_systems(gpu_ctx):
+    from hcgpu.abi import HcgpuError
+    F, p, q, _ = systems.four_bar()
+    with pytest.raises(HcgpuError, match="up to 16 variables"):
+        gpu_ctx.program(ParameterHomotopy(F, p, q), layout="group")
+
+
+def test_group_step_log_equals_oracle_log(gpu_ctx):
+    """The step records (HCG_LOG_* layout) written by the group layout equal the oracle's record
+    for record, on paths that reach the singular endgame (winding number 3)."""
+    H, S = total_degree(systems.hyperbolic(), seed=5)
+    S = S[:12]
+    g = track_batch(gpu_ctx.program(H, layout="group"), S, make_options(log_steps=1), log_capacity=50000)
+    o = oracle.track_batch(H, S, make_options(log_steps=1), log_capacity=50000)
+    assert_same(g, o)
+    gl, ol = g.log, o.log
+    gl = gl[np.lexsort((gl[:, 1], gl[:, 0]))]
+    ol = ol[np.lexsort((ol[:, 1], ol[:, 0]))]
+    assert gl.shape == ol.shape and len(gl) > 0
+    np.testing.assert_array_equal(gl, ol)
