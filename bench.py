@@ -68,20 +68,37 @@ def roofline(flops, kernel_ms, workload, kernel="hcg_track_kernel", info=None, g
                                        "workload / this launch's kernel time)",
                                "achieved_gbs": round(gbs, 1), "peak_gbs": HBM_PEAK_GBS,
                                "frac": round(gbs / HBM_PEAK_GBS, 4), "profile": prof.get("source")}
-    if "fp64_lane_ops" in prof:
-        r["fp64_counters"] = {"measured_fp64_lane_ops": prof["fp64_lane_ops"],
-                              "counted_flops_same_launch": prof.get("counted_flops"),
-                              "note": prof.get("fp64_note")}
+    if "fp64_wave_flops" in prof:
+        # SQ_INSTS_VALU_FLOPS_FP64 adds one lane's flops per wave instruction (ADD + MUL + 2 FMA +
+        # TRANS); times the 64 lanes of a wave it bounds the FP64 lane flops executed
+        bound = prof["fp64_wave_flops"] * 64
+        r["fp64_counters"] = {
+            "counter": prof["fp64_counter"], "per_launch": prof["fp64_wave_flops"],
+            "executed_lane_flops_upper_bound": bound,
+            "upper_bound_tflops": round(bound / (kernel_ms * 1e-3) / 1e12, 3),
+            "upper_bound_frac": round(bound / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+            "executed_bound_over_counted_flops": round(bound / flops, 3),
+            "profile": prof.get("counters_source")}
     if info is not None:
         occ = dict(info)
         if grid_waves is not None:
             occ["grid_waves"] = int(grid_waves)
             occ["resident_waves_per_simd"] = round(min(grid_waves, 4 * info["cus"] * info["waves_per_simd_built"])
                                                    / (4 * info["cus"]), 3)
-        if "sq_wave_cycles" in prof and "sq_busy_cycles" in prof:
-            occ["measured_mean_waves_per_simd"] = round(prof["sq_wave_cycles"] / prof["sq_busy_cycles"] / 4.0, 3)
+        if "mean_occupancy_per_cu" in prof:  # rocprofv3's MeanOccupancyPerCU of a profiled launch
+            occ["measured_mean_waves_per_simd"] = round(prof["mean_occupancy_per_cu"] / 4.0, 3)
         r["occupancy"] = occ
     return r
+
+
+def grid_waves(prog, npaths):
+    """Waves the launch starts.  Per-lane kernel: one per 64 paths, at most one per wave slot
+    (4 SIMDs x the waves per SIMD it was built for, per CU).  Lane-group kernel: four per
+    workgroup of 16 paths, at most two workgroups per CU."""
+    info = prog.kernel_info()
+    if getattr(prog, "layout", "lane") == "group":
+        return 4 * min(-(-npaths // 16), 2 * info["cus"])
+    return min(-(-npaths // 64), 4 * info["cus"] * info["waves_per_simd_built"])
 
 
 def host_info():
@@ -107,60 +124,10 @@ def native_oracle():
     return oracle, build
 
 
-class DeviceBatch:
+def DeviceBatch(*args, **kw):
     """A batch already resident in HBM (torch allocations), launched with hcg_track_batch_device."""
-
-    def __init__(self, ctx, prog, starts, p=None, Q=None, groups=None):
-        import numpy as np
-        import torch
-        from hcgpu.abi import hcg_batch, hcg_path_result
-        dev = torch.device("cuda:0")
-        self.npaths, self.n = starts.shape
-        self.ctx, self.prog = ctx, prog
-
-        def t(a):
-            return None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-
-        self.starts = t(np.ascontiguousarray(starts).view(np.float64))
-        self.p = t(None if p is None else np.ascontiguousarray(p).view(np.float64))
-        self.q = t(None if Q is None else np.ascontiguousarray(Q).view(np.float64))
-        self.g = t(groups)
-        self.res = torch.empty(self.npaths * ctypes.sizeof(hcg_path_result), dtype=torch.uint8, device=dev)
-        self.sol = torch.empty(self.npaths * self.n * 2, dtype=torch.float64, device=dev)
-        self.last = torch.empty_like(self.sol)
-        self.val = torch.empty(self.npaths * self.n, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
-        b = hcg_batch()
-        b.npaths = self.npaths
-        P = ctypes.POINTER(ctypes.c_double)
-        b.starts = ctypes.cast(self.starts.data_ptr(), P)
-        b.t1, b.t0 = 1.0, 0.0
-        if self.p is not None:
-            b.start_params = ctypes.cast(self.p.data_ptr(), P)
-            b.target_params = ctypes.cast(self.q.data_ptr(), P)
-            b.ngroups = Q.shape[0]
-            b.group_of_path = ctypes.cast(self.g.data_ptr(), ctypes.POINTER(ctypes.c_int32))
-        self.b = b
-
-    def launch(self, opts):
-        from hcgpu import abi
-        L = abi.lib()
-        L.hcg_track_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(abi.hcg_batch),
-                                             ctypes.POINTER(abi.hcg_options), ctypes.c_void_p, ctypes.c_void_p,
-                                             ctypes.c_void_p, ctypes.c_void_p]
-        abi.check(L.hcg_track_batch_device(self.ctx.handle, self.prog._h, ctypes.byref(self.b), ctypes.byref(opts),
-                                           self.res.data_ptr(), self.sol.data_ptr(), self.last.data_ptr(),
-                                           self.val.data_ptr()))
-        return self.ctx.last_kernel_ms(), self.ctx.last_counters()
-
-    def results(self):
-        import numpy as np
-        from hcgpu.abi import hcg_path_result
-        dt = np.dtype([(f, np.int32 if t is ctypes.c_int32 else np.float64) for f, t in hcg_path_result._fields_],
-                      align=True)
-        rec = np.frombuffer(self.res.cpu().numpy().tobytes(), dtype=dt)
-        sol = self.sol.cpu().numpy().view(np.complex128).reshape(self.npaths, self.n)
-        return rec, sol
+    from hcgpu.device import DeviceBatch as _DeviceBatch
+    return _DeviceBatch(*args, **kw)
 
 
 def timed(batch, opts, steps, warmup):
@@ -286,9 +253,63 @@ def training_leg(ctx, F, p0, S, args):
     return leg
 
 
+# per-lane (waves per SIMD, workgroups per CU) candidates the benchmark's autotuning pass times
+BENCH_LANE_CANDIDATES = ((1, 8), (2, 8), (2, 16), (4, 16))
+
+
+def graph_leg(ctx, prog, batch, opts, rl, rec_l, sol_l, H, S, args):
+    """The same launch captured once as a HIP graph (hcg_graph_create) and replayed: the whole
+    batch, and a one-target batch (1,024 paths) where launch overhead is a larger share."""
+    import numpy as np
+    import torch
+
+    def same(a, b):
+        return bool(np.array_equal(a[0].view(np.uint8), b[0].view(np.uint8)) and
+                    np.array_equal(a[1].view(np.uint8), b[1].view(np.uint8)))
+
+    def replay(g, k):
+        kms = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            kms.append(g.launch()[0])
+        return (time.perf_counter() - t0) / k * 1e3, float(np.mean(kms))
+
+    leg = {}
+    g = batch.graph(opts)
+    try:
+        g.launch()
+        wall_ms, kms = replay(g, args.steps)
+        rec_g, sol_g = batch.results()[:2]
+        leg["full_batch"] = {"paths": batch.npaths, "direct_ms_per_batch": round(rl["wall"] / args.steps * 1e3, 3),
+                             "graph_ms_per_batch": round(wall_ms, 3), "graph_kernel_ms": round(kms, 3),
+                             "identical_to_direct": same((rec_g, sol_g), (rec_l, sol_l))}
+    finally:
+        g.close()
+    small = DeviceBatch(ctx, prog, S, H.p, H.q[:1], np.zeros(len(S), dtype=np.int32))
+    k = max(20, args.steps)
+    small.launch(opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        small.launch(opts)
+    direct_ms = (time.perf_counter() - t0) / k * 1e3
+    ref = small.results()
+    g = small.graph(opts)
+    try:
+        g.launch()
+        wall_ms, kms = replay(g, k)
+        leg["one_target_batch"] = {"paths": small.npaths, "direct_ms_per_batch": round(direct_ms, 3),
+                                   "graph_ms_per_batch": round(wall_ms, 3), "graph_kernel_ms": round(kms, 3),
+                                   "identical_to_direct": same(small.results(), ref)}
+    finally:
+        g.close()
+    return leg
+
+
 def bench_param10(args, out):
     import numpy as np
-    from hcgpu import learned
+    from hcgpu import autotune, learned
     from hcgpu.configs import cfg2_problem
     from hcgpu.gpu import GpuContext
     from hcgpu.homotopies import ParameterHomotopy
@@ -299,17 +320,31 @@ def bench_param10(args, out):
     starts = np.tile(S, (len(Q), 1))
     groups = np.repeat(np.arange(len(Q), dtype=np.int32), len(S))
     ctx = GpuContext(0)
-    prog = ctx.program(H, layout=args.layout)
-    batch = DeviceBatch(ctx, prog, starts, H.p, H.q, groups)
-    npaths = len(starts)
-
     w, wsrc = learned_weights(ctx, F, p0, S)
     learned.load(ctx, w)
     plain_o, learned_o = make_options(), make_options(learned_step=1)
+    if args.layout == "auto":
+        # the autotuner on the measured configuration (learned step, whole batch), before the
+        # timed region; its choice is cached per device and GpuContext.program() builds it
+        cands = autotune.candidates_for(H.n, lane=BENCH_LANE_CANDIDATES, group=(2,))
+        t0 = time.perf_counter()
+        tuned = autotune.tune(ctx, H, starts, learned_o, groups, candidates=cands, force=args.retune)
+        tune_s = time.perf_counter() - t0
+        prog = ctx.program(H)
+        launch = {"chosen_by": "autotuner", "tuning_s": round(tune_s, 1), "table": tuned["table"]}
+    else:
+        prog = ctx.program(H, layout=args.layout)
+        launch = {"chosen_by": "--layout"}
+    plain_o.blocks_per_cu = learned_o.blocks_per_cu = prog.blocks_per_cu
+    info = prog.kernel_info()
+    launch.update({"layout": prog.layout, "waves_per_simd_built": info["waves_per_simd_built"],
+                   "blocks_per_cu": prog.blocks_per_cu or None})
+    batch = DeviceBatch(ctx, prog, starts, H.p, H.q, groups)
+    npaths = len(starts)
     rp = timed(batch, plain_o, args.steps, args.warmup)
-    rec_p, sol_p = batch.results()
+    rec_p, sol_p = batch.results()[:2]
     rl = timed(batch, learned_o, args.steps, args.warmup)
-    rec_l, sol_l = batch.results()
+    rec_l, sol_l = batch.results()[:2]
 
     value = npaths * args.steps / rl["wall"]
     plain_value = npaths * args.steps / rp["wall"]
@@ -323,10 +358,11 @@ def bench_param10(args, out):
                                "1024 starts x 256 targets = 262144 paths, one persistent launch, learned-step tracker",
                    "n": 10, "paths": npaths, "starts": len(S), "targets": len(Q), "params": 16,
                    "affine_parameter_lift": True, "weights": wsrc,
-                   "layout": "one path per lane" if args.layout == "lane" else "one path per 16-lane group"},
-        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned" + ("" if args.layout == "lane" else "_group"),
-                             kernel="hcg_track_kernel" if args.layout == "lane" else "hcg_group_kernel",
-                             info=prog.kernel_info(), grid_waves=-(-npaths // 64)),
+                   "layout": "one path per lane" if prog.layout == "lane" else "one path per 16-lane group",
+                   "launch": launch},
+        "roofline": roofline(rl["flops"], rl["kernel_ms"], "param10_learned" + ("" if prog.layout == "lane" else "_group"),
+                             kernel="hcg_track_kernel" if prog.layout == "lane" else "hcg_group_kernel",
+                             info=prog.kernel_info(), grid_waves=grid_waves(prog, npaths)),
         "plain_gpu": {"value": round(plain_value, 1), "unit": "paths/s", "kernel_ms": round(rp["kernel_ms"], 3),
                       "steps_per_path": round(rp["steps"] / npaths, 3),
                       "roofline_frac": round(rp["flops"] / (rp["kernel_ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5)},
@@ -339,6 +375,8 @@ def bench_param10(args, out):
         "parity_learned_vs_plain": {"identical_return_codes": bool((rec_l["code"] == rec_p["code"]).all()),
                                     "max_rel_endpoint_diff": float(rel.max())},
     })
+    if not args.no_graph:
+        out["graph_replay"] = graph_leg(ctx, prog, batch, learned_o, rl, rec_l, sol_l, H, S, args)
     if not args.no_certify:
         out["certification"] = certify_leg(ctx, F, Q, sol_l, rec_l, groups, args)
     if not args.no_training:
@@ -375,16 +413,16 @@ def bench_dense8(args, out):
     ctx = GpuContext(0)
     # 6,561 paths fill 103 waves, far fewer than the GPU's SIMDs: the kernel is built for one
     # wave per SIMD (the largest register budget) instead of the default four
-    prog = ctx.program(H, waves_per_eu=1) if args.layout == "lane" else ctx.program(H, layout="group")
+    prog = ctx.program(H, waves_per_eu=1) if args.layout != "group" else ctx.program(H, layout="group")
     batch = DeviceBatch(ctx, prog, S)
     r = timed(batch, make_options(), args.steps, args.warmup)
-    rec, _ = batch.results()
+    rec = batch.results()[0]
     npaths = len(S)
     out.update({"value": round(npaths * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
                 "config": {"workload": "dense8: n=8, 8 dense cubics (165 monomials each), 6561 total-degree paths, "
                                        "one persistent launch, plain tracker", "n": 8, "paths": npaths},
                 "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths", info=prog.kernel_info(),
-                                     grid_waves=-(-npaths // 64)),
+                                     grid_waves=grid_waves(prog, npaths)),
                 "steps_per_path": round(r["steps"] / npaths, 3), "solution_counts": counts(rec)})
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
@@ -408,7 +446,7 @@ def bench_mixed(args, out):
     from hcgpu.solve import make_options
     probs = cfg4_problem()
     ctx = GpuContext(0)
-    prog = ctx.program_multi([H for H, _ in probs], layout=args.layout)
+    prog = ctx.program_multi([H for H, _ in probs], layout="group" if args.layout == "group" else "lane")
     starts = [S for _, S in probs]
     total = sum(len(S) for S in starts)
     learned.load(ctx, learned.saved_weights("mixed_step_mlp"))
@@ -432,7 +470,7 @@ def bench_mixed(args, out):
                                        "from the host each launch)", "paths": total,
                            "weights": "committed (hcgpu/weights/mixed_step_mlp.npy)"},
                 "roofline": roofline(lfl, lms, "mixed_learned", kernel="hcg_track_multi_kernel",
-                                     info=prog.kernel_info(), grid_waves=-(-total // 64)),
+                                     info=prog.kernel_info(), grid_waves=grid_waves(prog, total)),
                 "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
                 "learned": {"speedup_vs_plain": round(pms / lms, 3)},
                 "identical_return_codes_learned_vs_plain": same,
@@ -467,14 +505,17 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
-    ap.add_argument("--layout", choices=["lane", "group"], default="lane",
-                    help="lane: one path per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
+    ap.add_argument("--retune", action="store_true", help="tune again even when the autotuner has a cached choice")
+    ap.add_argument("--layout", choices=["auto", "lane", "group"], default="auto",
+                    help="auto: the autotuner's choice (param10; the other workloads take lane); lane: one path "
+                         "per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
     ap.add_argument("--no-training", action="store_true", help="skip the proposer-training leg (param10)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg (param10)")
     ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
                     help="threads of the CPU baseline (default: every CPU this process may use)")
-    ap.add_argument("--cpu-targets", type=int, default=32, help="targets of the CPU baseline sample (param10)")
+    ap.add_argument("--cpu-targets", type=int, default=192, help="targets of the CPU baseline sample (param10)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run one plain device-to-device copy of this many MiB (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counter readings) and exit")

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -466,12 +467,12 @@ static int ctx_scalars(hcg_ctx* ctx) {
 // Launch the lane-group kernel: 16 paths per workgroup of 256 lanes, at most
 // blocks_per_cu workgroups per CU (default 2).
 static int group_launch(hcg_ctx* ctx, const hcg_program* prog, const hcg_multi_kargs& m, long total,
-                        const hcg_options* opts) {
+                        const hcg_options* opts, hipStream_t stream) {
     const int per_block = hcg::group_block() / HCG_GROUP_LANES;
     const int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 2;
     long grid = std::min<long>((total + per_block - 1) / per_block, (long)ctx->ncu * per_cu);
     if (grid < 1) grid = 1;
-    hipError_t e = hcg::group_launch(prog->group->nn, (int)grid, prog->group->lds_per_path, ctx->stream, m,
+    hipError_t e = hcg::group_launch(prog->group->nn, (int)grid, prog->group->lds_per_path, stream, m,
                                      prog->group->d_sys);
     if (e != hipSuccess) return fail(HCG_ERR_HIP, "group kernel launch: %s", hipGetErrorString(e));
     return HCG_OK;
@@ -496,16 +497,18 @@ static int ctx_launch_state(hcg_ctx* ctx, LaunchState* st) {
     return HCG_OK;
 }
 
-// Launch the persistent tracker kernel over device-resident inputs/outputs.
-static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
-                        hcg_path_result* d_res, double* d_sol, double* d_last, double* d_val, float* d_log, int logcap,
-                        int32_t* d_logc, const LaunchState& st) {
+// Enqueue one launch on `stream`: the reset of the launch state (queue head, counters, options)
+// and the persistent tracker kernel over device-resident inputs/outputs.  `timed` records the
+// context's start event right before the kernel (a captured graph is timed around its replay).
+static int enqueue_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                         hcg_path_result* d_res, double* d_sol, double* d_last, double* d_val, float* d_log, int logcap,
+                         int32_t* d_logc, const LaunchState& st, hipStream_t stream, bool timed) {
     if (st.fixed) {
-        if (hcg::noalloc_init(ctx->stream, st.fixed, opts)) return fail(HCG_ERR_HIP, "scratch init kernel launch failed");
+        if (hcg::noalloc_init(stream, st.fixed, opts)) return fail(HCG_ERR_HIP, "launch-state init kernel failed");
     } else {
-        HIPCHK(hipMemcpyAsync(st.opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemsetAsync(st.next, 0, sizeof(int32_t), ctx->stream));
-        HIPCHK(hipMemsetAsync(st.counters, 0, sizeof(unsigned long long) * 4, ctx->stream));
+        HIPCHK(hipMemcpyAsync(st.opts, opts, sizeof(hcg_options), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemsetAsync(st.next, 0, sizeof(int32_t), stream));
+        HIPCHK(hipMemsetAsync(st.counters, 0, sizeof(unsigned long long) * 4, stream));
     }
     hcg_kargs ka;
     std::memset(&ka, 0, sizeof ka);
@@ -528,6 +531,7 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
     ka.log_records = d_log;
     ka.log_count = d_logc;
     ka.mlp = opts->learned_step ? ctx->d_mlp : nullptr;
+    if (timed) HIPCHK(hipEventRecord(ctx->ev0, stream));
     if (prog->group) {
         hcg_multi_kargs m;
         std::memset(&m, 0, sizeof m);
@@ -537,19 +541,26 @@ static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* 
         m.offset[1] = b->npaths;
         m.next_path = st.next;
         m.sys[0] = ka;
-        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        int rc = group_launch(ctx, prog, m, b->npaths, opts);
-        if (rc) return rc;
-    } else {
-        const int block = 64;
-        int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
-        long waves_needed = ((long)b->npaths + block - 1) / block;
-        long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
-        if (grid < 1) grid = 1;
-        void* args[] = {&ka};
-        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+        return group_launch(ctx, prog, m, b->npaths, opts, stream);
     }
+    const int block = 64;
+    int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 4 * prog->waves_per_eu;  // every wave slot of the CU's 4 SIMDs
+    long waves_needed = ((long)b->npaths + block - 1) / block;
+    long grid = std::min<long>(waves_needed, (long)ctx->ncu * per_cu);
+    if (grid < 1) grid = 1;
+    void* args[] = {&ka};
+    HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, stream, args, nullptr));
+    return HCG_OK;
+}
+
+// Launch the persistent tracker kernel over device-resident inputs/outputs on the context's
+// stream, timed by the context's events; the launch counters are copied back after it.
+static int launch_track(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* b, const hcg_options* opts,
+                        hcg_path_result* d_res, double* d_sol, double* d_last, double* d_val, float* d_log, int logcap,
+                        int32_t* d_logc, const LaunchState& st) {
+    int rc = enqueue_track(ctx, prog, b, opts, d_res, d_sol, d_last, d_val, d_log, logcap, d_logc, st, ctx->stream,
+                           true);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->last_counters, st.counters, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -671,6 +682,71 @@ static int hcg_track_batch_device_noalloc_impl(hcg_ctx* ctx, const hcg_program* 
     return finish_launch(ctx);
 }
 
+// A captured device-resident launch: the launch-state reset kernel and the tracker kernel as
+// one HIP graph, replayed by hcg_graph_launch.  The graph owns its launch state (queue head,
+// counters, options), cleared by the reset kernel at every replay (no memset node).
+struct hcg_graph {
+    hcg_ctx* ctx = nullptr;
+    const hcg_program* prog = nullptr;
+    hcg::NoallocScratch* state = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void graph_free(hcg_graph* g) {
+    if (!g) return;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->state) hipFree(g->state);
+    delete g;
+}
+
+static int hcg_graph_create_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                 const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
+                                 double* last_points_dev, double* valuations_dev, hcg_graph** out) {
+    if (!out) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL out");
+    *out = nullptr;
+    int rc = validate_batch_args(ctx, prog, batch_dev, opts, false);
+    if (rc) return rc;
+    if (!results_dev || !solutions_dev || !last_points_dev || !valuations_dev)
+        return fail(HCG_ERR_INVALID_ARGUMENT, "NULL output");
+    if (batch_dev->npaths < 1) return fail(HCG_ERR_INVALID_ARGUMENT, "a graph needs at least one path");
+    HIPCHK(hipSetDevice(ctx->device));
+    std::unique_ptr<hcg_graph, void (*)(hcg_graph*)> g(new hcg_graph, graph_free);
+    g->ctx = ctx;
+    g->prog = prog;
+    HIPCHK(hipMalloc((void**)&g->state, sizeof(hcg::NoallocScratch)));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    LaunchState st;
+    st.next = &g->state->next_path;
+    st.counters = g->state->counters;
+    st.opts = &g->state->opts;
+    st.fixed = g->state;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_track(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev,
+                       nullptr, 0, nullptr, st, ctx->stream, false);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    g->graph = graph;
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(HCG_ERR_HIP, "stream capture of the launch: %s", hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return HCG_OK;
+}
+
+static int hcg_graph_launch_impl(hcg_graph* g) {
+    if (!g || !g->exec) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL graph");
+    hcg_ctx* ctx = g->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipGraphLaunch(g->exec, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->last_counters, g->state->counters, sizeof(unsigned long long) * 4,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    return finish_launch(ctx);
+}
+
 static int hcg_track_multi_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches,
                                const hcg_options* opts, hcg_path_result* results, double* solutions,
                                double* last_points, double* valuations) {
@@ -779,7 +855,7 @@ static int hcg_track_multi_impl(hcg_ctx* ctx, const hcg_program* prog, const hcg
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (prog->group) {
         for (int s = 0; s < S; s++) m.sys[s].counters = ctx->d_counters, m.sys[s].opts = ctx->d_opts;
-        rc = group_launch(ctx, prog, m, total, opts);
+        rc = group_launch(ctx, prog, m, total, opts, ctx->stream);
         if (rc) return rc;
     } else {
         HIPCHK(hipModuleLaunchKernel(prog->kernel, (unsigned)grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
@@ -947,6 +1023,26 @@ extern "C" int hcg_track_batch_device_noalloc(hcg_ctx* ctx, const hcg_program* p
         return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_track_batch_device_noalloc: unexpected error");
     }
 }
+extern "C" int hcg_graph_create(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev,
+                                const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
+                                double* last_points_dev, double* valuations_dev, hcg_graph** out) {
+    try {
+        return hcg_graph_create_impl(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev,
+                                     valuations_dev, out);
+    } catch (const std::exception& ex) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_graph_create: %s", ex.what());
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_graph_create: unexpected error");
+    }
+}
+extern "C" int hcg_graph_launch(hcg_graph* g) {
+    try {
+        return hcg_graph_launch_impl(g);
+    } catch (...) {
+        return fail(HCG_ERR_INVALID_ARGUMENT, "hcg_graph_launch: unexpected error");
+    }
+}
+extern "C" void hcg_graph_destroy(hcg_graph* g) { graph_free(g); }
 extern "C" int hcg_track_multi(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batches, const hcg_options* opts, hcg_path_result* results, double* solutions, double* last_points, double* valuations) {
     try {
         return hcg_track_multi_impl(ctx, prog, batches, opts, results, solutions, last_points, valuations);

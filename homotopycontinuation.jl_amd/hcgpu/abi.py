@@ -16,6 +16,7 @@ LIB_PATH = PKG_ROOT / "_build" / "libhcgpu.so"
 
 HCG_STRAIGHT_LINE = 0
 HCG_PARAMETER = 1
+HCG_OK, HCG_ERR_INVALID_ARGUMENT, HCG_ERR_COMPILE, HCG_ERR_HIP, HCG_ERR_OUT_OF_MEMORY, HCG_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 HCG_STEP_RECORD_FLOATS = 16
 HCG_MLP_IN, HCG_MLP_HIDDEN, HCG_MLP_OUT = 8, 64, 2
 HCG_MLP_NPARAMS = (HCG_MLP_IN * HCG_MLP_HIDDEN + HCG_MLP_HIDDEN + HCG_MLP_HIDDEN * HCG_MLP_HIDDEN
@@ -210,6 +211,13 @@ def lib():
     L.hcg_certifier_compile_only.argtypes = [P(hcg_system), _I, P(ctypes.c_size_t)]
     L.hcg_certify.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _I, P(_D), _I, P(_D), P(_D), P(_D), P(_I), _I, _I,
                               _I, P(hcg_certificate), P(_D), P(_D), P(_D)]
+    L.hcg_track_batch_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(hcg_batch), P(hcg_options),
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.hcg_graph_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(hcg_batch), P(hcg_options), ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_void_p)]
+    L.hcg_graph_launch.argtypes = [ctypes.c_void_p]
+    L.hcg_graph_destroy.argtypes = [ctypes.c_void_p]
+    L.hcg_graph_destroy.restype = None
     _lib = L
     return L
 
@@ -227,5 +235,5 @@ HCGPU_SYMBOLS = [
     "hcg_last_counters", "hcg_track_batch_device", "hcg_program_create_multi", "hcg_compile_multi_only",
     "hcg_track_multi", "hcg_program_create_ex", "hcg_compile_ex", "hcg_certifier_create",
     "hcg_certifier_compile_only", "hcg_certify", "hcg_program_kernel_info", "hcg_track_batch_device_noalloc",
-    "hcg_group_schedule_info",
+    "hcg_group_schedule_info", "hcg_graph_create", "hcg_graph_launch", "hcg_graph_destroy",
 ]

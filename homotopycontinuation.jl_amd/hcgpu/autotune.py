@@ -1,11 +1,14 @@
-"""Autotuner of the tracker kernel's build and launch shape.
+"""Autotuner of the tracker's layout, build and launch shape.
 
-For a homotopy and a representative sample of its batch, times every candidate
-(waves per SIMD the kernel is compiled for = its register budget, x workgroups per CU of
-the persistent launch) and keeps the fastest.  Choices are cached in a JSON file next to
-libhcgpu.so, keyed by the program's generated source and the device, so a workload is
-tuned once.  Tuning never changes results: every candidate runs the same arithmetic; the
-tuner checks that the sample's return codes agree across candidates.
+For a homotopy and a representative sample of its batch, times every candidate and keeps the
+fastest.  A candidate is a layout (one path per lane, or one path per 16-lane group for
+systems of up to 16 variables), the waves per SIMD the per-lane kernel is compiled for (its
+register budget), and the workgroups per CU of the persistent launch.  Choices are cached in a
+JSON file next to libhcgpu.so, keyed by the program's generated source and the device, so a
+workload is tuned once; GpuContext.program() and track_batch() then use the cached choice by
+default.  Tuning never changes results: every candidate runs the same arithmetic (both layouts
+equal the oracle bit for bit), and the tuner checks that the sample's return codes and
+endpoints agree across candidates.
 """
 from __future__ import annotations
 
@@ -13,7 +16,7 @@ import ctypes
 import hashlib
 import json
 from pathlib import Path
-from typing import Dict, Iterable, Optional, Sequence, Tuple
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -21,6 +24,24 @@ from . import abi
 from .gpu import GpuContext, track_batch
 
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = ((1, 4), (1, 8), (2, 8), (2, 16), (4, 16), (4, 32), (8, 32))
+GROUP_CANDIDATES: Tuple[int, ...] = (1, 2)  # workgroups per CU of the lane-group kernel
+GROUP_MAX_N = 16
+
+
+def candidates_for(n: int, lane=DEFAULT_CANDIDATES, group=GROUP_CANDIDATES) -> List[dict]:
+    """Every (layout, waves_per_eu, blocks_per_cu) candidate for a system of n variables."""
+    out = [dict(layout="lane", waves_per_eu=w, blocks_per_cu=b) for w, b in lane]
+    if n <= GROUP_MAX_N:
+        out += [dict(layout="group", waves_per_eu=0, blocks_per_cu=b) for b in group]
+    return out
+
+
+def _as_candidate(c) -> dict:
+    if isinstance(c, dict):
+        return dict(layout=c.get("layout", "lane"), waves_per_eu=int(c.get("waves_per_eu", 0)),
+                    blocks_per_cu=int(c.get("blocks_per_cu", 0)))
+    w, b = c
+    return dict(layout="lane", waves_per_eu=int(w), blocks_per_cu=int(b))
 
 
 def _cache_path() -> Path:
@@ -57,36 +78,46 @@ def _store(key: str, entry: dict) -> None:
 
 
 def tune(ctx: GpuContext, homotopy, starts, opts: Optional[abi.hcg_options] = None, group_of_path=None,
-         candidates: Iterable[Tuple[int, int]] = DEFAULT_CANDIDATES, force: bool = False) -> dict:
-    """Best {waves_per_eu, blocks_per_cu} for tracking `starts` (a representative sample)
-    with `homotopy`; cached unless force=True."""
+         candidates: Optional[Iterable] = None, force: bool = False) -> dict:
+    """Best {layout, waves_per_eu, blocks_per_cu} for tracking `starts` (a representative
+    sample) with `homotopy`; cached unless force=True.  candidates: dicts as candidates_for()
+    returns, or (waves_per_eu, blocks_per_cu) pairs of the per-lane layout; default: every
+    candidate for the homotopy's size."""
     key = _key(ctx, [homotopy])
     if not force:
         hit = _load().get(key)
         if hit is not None:
-            return hit
+            return _normalized(hit)
     opts = opts or abi.default_options()
+    cands = [_as_candidate(c) for c in (candidates if candidates is not None else candidates_for(homotopy.n))]
     table = []
-    ref_codes = None
-    for w, bpc in candidates:
-        prog = ctx.program(homotopy, waves_per_eu=w)
+    ref = None
+    for c in cands:
+        prog = ctx.program(homotopy, waves_per_eu=c["waves_per_eu"], layout=c["layout"])
         o = abi.hcg_options.from_buffer_copy(opts)
-        o.blocks_per_cu = bpc
+        o.blocks_per_cu = c["blocks_per_cu"]
         track_batch(prog, starts, o, group_of_path=group_of_path)  # warm
         out = track_batch(prog, starts, o, group_of_path=group_of_path)
         ms = ctx.last_kernel_ms()
-        if ref_codes is None:
-            ref_codes = out.code.copy()
-        elif not np.array_equal(ref_codes, out.code):
-            raise RuntimeError(f"return codes differ between launch shapes (waves_per_eu={w}, blocks_per_cu={bpc})")
-        table.append({"waves_per_eu": w, "blocks_per_cu": bpc, "kernel_ms": round(ms, 3)})
+        if ref is None:
+            ref = out
+        elif not (np.array_equal(ref.code, out.code) and np.array_equal(ref.solution, out.solution, equal_nan=True)):
+            raise RuntimeError(f"results differ between launch shapes ({c})")
+        table.append(dict(c, kernel_ms=round(ms, 3)))
     best = min(table, key=lambda r: r["kernel_ms"])
-    entry = {"waves_per_eu": best["waves_per_eu"], "blocks_per_cu": best["blocks_per_cu"], "sample_paths": len(starts),
-             "table": table}
+    entry = {"layout": best["layout"], "waves_per_eu": best["waves_per_eu"], "blocks_per_cu": best["blocks_per_cu"],
+             "sample_paths": len(starts), "table": table}
     _store(key, entry)
     return entry
 
 
+def _normalized(entry: dict) -> dict:
+    e = dict(entry)
+    e.setdefault("layout", "lane")
+    return e
+
+
 def tuned_choice(ctx: GpuContext, homotopy) -> Optional[dict]:
     """The cached choice for `homotopy` on this device, if it was tuned."""
-    return _load().get(_key(ctx, [homotopy]))
+    hit = _load().get(_key(ctx, [homotopy]))
+    return None if hit is None else _normalized(hit)

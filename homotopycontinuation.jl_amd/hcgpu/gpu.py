@@ -19,20 +19,38 @@ class GpuContext:
         check(lib().hcg_ctx_create(device, ctypes.byref(self._h)))
         self.device = device
         self._programs: Dict[int, "Program"] = {}
+        self._tuned: Dict[int, dict] = {}  # autotuner choices looked up per homotopy
 
     @property
     def handle(self):
         return self._h
 
-    def program(self, homotopy, waves_per_eu: int = 0, layout: str = "lane") -> "Program":
+    def program(self, homotopy, waves_per_eu: int = 0, layout: Optional[str] = None) -> "Program":
         """layout "lane": one path per lane, the tapes compiled with hipRTC; "group": one path
         per group of 16 lanes on the precompiled lane-group kernel (systems of up to 16
-        variables)."""
+        variables).  With neither layout nor waves_per_eu given, the autotuner's cached choice
+        for this homotopy on this device (hcgpu.autotune.tune) when there is one, including
+        its workgroups per CU, which track_batch then uses; otherwise one path per lane at the
+        library's default register budget."""
+        blocks_per_cu = 0
+        tuned = None
+        if layout is None and waves_per_eu == 0:
+            tuned = self._tuned.get(id(homotopy))
+            if tuned is None:
+                from . import autotune
+                tuned = autotune.tuned_choice(self, homotopy) or {}
+                self._tuned[id(homotopy)] = tuned
+            if tuned:
+                layout, waves_per_eu, blocks_per_cu = tuned["layout"], tuned["waves_per_eu"], tuned["blocks_per_cu"]
+        layout = layout or "lane"
         key = (id(homotopy), waves_per_eu, layout)
         p = self._programs.get(key)
         if p is None:
             p = Program(self, homotopy, waves_per_eu, layout)
             self._programs[key] = p
+        if tuned:
+            p.tuned = dict(tuned)
+            p.blocks_per_cu = blocks_per_cu
         return p
 
     def program_multi(self, homotopies, waves_per_eu: int = 0, layout: str = "lane") -> "MultiProgram":
@@ -132,6 +150,8 @@ class Program:
         self.n = homotopy.n
         self.waves_per_eu = waves_per_eu
         self.layout = layout
+        self.blocks_per_cu = 0  # launch shape from the autotuner (0: the library's default)
+        self.tuned = None
         self._h = _create(ctx, [homotopy.desc()], False, waves_per_eu, layout)
 
     def source(self) -> str:
@@ -179,7 +199,9 @@ def track_batch(program: Program, starts, opts: Optional[hcg_options] = None, ta
     """Track all `starts` (npaths x n complex) from t=1 to t=0 on the GPU.  For a parameter
     homotopy, `start_params` / `target_groups` replace the homotopy's own p / q (in the
     space its tapes use, see ParameterHomotopy.target_matrix)."""
-    opts = opts or abi.default_options()
+    opts = hcg_options.from_buffer_copy(opts) if opts is not None else abi.default_options()
+    if opts.blocks_per_cu == 0 and getattr(program, "blocks_per_cu", 0):
+        opts.blocks_per_cu = program.blocks_per_cu  # the autotuner's launch shape
     starts = np.ascontiguousarray(np.asarray(starts, dtype=np.complex128))
     npaths, n = starts.shape
     if n != program.n:

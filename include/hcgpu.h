@@ -143,6 +143,25 @@ int hcg_track_batch_device_noalloc(hcg_ctx* ctx, const hcg_program* prog, const 
                                    const hcg_options* opts, hcg_path_result* results_dev, double* solutions_dev,
                                    double* last_points_dev, double* valuations_dev);
 
+/* Steady-state relaunch of a device-resident batch as one HIP graph.  No counterpart in the
+ * reference, which re-enters its per-path loop for every batch (many_solve,
+ * src/solve.jl:815-881); this is for callers that track batch after batch through the same
+ * buffers (one parameter set per batch, a serving loop).  hcg_graph_create captures what
+ * hcg_track_batch_device enqueues for these arguments: a kernel that clears the graph's own
+ * launch state (work-queue head, counters, options; no memset node) and the tracker kernel.
+ * The graph keeps every pointer and the options as given: the caller rewrites the same device
+ * buffers (new starts, new target parameters, same path count) and calls hcg_graph_launch, which
+ * replays both kernels on the context's stream and returns when they are done (kernel time and
+ * counters as for hcg_last_kernel_stats / hcg_last_counters).  Results equal those of
+ * hcg_track_batch_device on the same inputs.  Destroy the graph before its program and
+ * context. */
+typedef struct hcg_graph hcg_graph;
+int hcg_graph_create(hcg_ctx* ctx, const hcg_program* prog, const hcg_batch* batch_dev, const hcg_options* opts,
+                     hcg_path_result* results_dev, double* solutions_dev, double* last_points_dev,
+                     double* valuations_dev, hcg_graph** out);
+int hcg_graph_launch(hcg_graph* g);
+void hcg_graph_destroy(hcg_graph* g);
+
 /* Several homotopies (1..8, any sizes and kinds) compiled into one program: one
  * persistent launch tracks the paths of all of them from a single work queue.  This is the
  * batch form of calling threaded_solve (src/solve.jl:628-709) once per system; there is

@@ -206,6 +206,7 @@ end
 # ---------------------------------------------------------------------------------------------
 const Ctx = Ptr{Cvoid}       # hcg_ctx*
 const Prog = Ptr{Cvoid}      # hcg_program*
+const Graph = Ptr{Cvoid}     # hcg_graph*
 
 hcg_ctx_create(device, out) = ccall((:hcg_ctx_create, libhcgpu), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, out)
 hcg_ctx_destroy(ctx) = ccall((:hcg_ctx_destroy, libhcgpu), Cvoid, (Ptr{Cvoid},), ctx)
@@ -230,6 +231,13 @@ hcg_track_batch_device_noalloc(ctx, prog, batch_dev, opts, results_dev, solution
           (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{HcgBatch}, Ptr{HcgOptions}, Ptr{HcgPathResult}, Ptr{Float64},
            Ptr{Float64}, Ptr{Float64}),
           ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev)
+hcg_graph_create(ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, out) =
+    ccall((:hcg_graph_create, libhcgpu), Cint,
+          (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{HcgBatch}, Ptr{HcgOptions}, Ptr{HcgPathResult}, Ptr{Float64},
+           Ptr{Float64}, Ptr{Float64}, Ptr{Ptr{Cvoid}}),
+          ctx, prog, batch_dev, opts, results_dev, solutions_dev, last_points_dev, valuations_dev, out)
+hcg_graph_launch(g) = ccall((:hcg_graph_launch, libhcgpu), Cint, (Ptr{Cvoid},), g)
+hcg_graph_destroy(g) = ccall((:hcg_graph_destroy, libhcgpu), Cvoid, (Ptr{Cvoid},), g)
 hcg_program_create_multi(ctx, hs, nsys, out) =
     ccall((:hcg_program_create_multi, libhcgpu), Cint,
           (Ptr{Cvoid}, Ptr{Ptr{HcgHomotopyDesc}}, Int32, Ptr{Ptr{Cvoid}}), ctx, hs, nsys, out)

@@ -134,6 +134,18 @@ def test_parameter_homotopy_requires_parameters():
     assert b"parameter" in L.hcg_last_error()
 
 
+def test_graph_entry_points_reject_missing_arguments():
+    """hcg_graph_create / hcg_graph_launch validate before touching a device; destroy of NULL is a no-op."""
+    L = abi.lib()
+    out = ctypes.c_void_p()
+    rc = L.hcg_graph_create(None, None, None, None, None, None, None, None, ctypes.byref(out))
+    assert rc == abi.HCG_ERR_INVALID_ARGUMENT and not out.value
+    assert L.hcg_graph_create(None, None, None, None, None, None, None, None, None) == abi.HCG_ERR_INVALID_ARGUMENT
+    assert L.hcg_graph_launch(None) == abi.HCG_ERR_INVALID_ARGUMENT
+    assert b"graph" in L.hcg_last_error()
+    L.hcg_graph_destroy(None)
+
+
 def test_no_gpu_means_a_loud_failure():
     """The product path has no CPU fallback: without a device the context cannot be made."""
     import torch

@@ -18,7 +18,7 @@ JULIA = REPO / "julia" / "HCGPU.jl"
 SCALARS = {"int32_t": "Int32", "double": "Float64", "float": "Float32", "uint64_t": "UInt64",
            "int64_t": "Int64", "size_t": "Csize_t", "long": "Clong", "int": "Cint", "char": "UInt8",
            "void": "Cvoid"}
-OPAQUE = {"hcg_ctx", "hcg_program"}
+OPAQUE = {"hcg_ctx", "hcg_program", "hcg_graph"}
 
 
 def strip_c_comments(text):
