@@ -499,6 +499,21 @@ def bench_mixed(args, out):
         out["gpu_vs_cpu"] = round(out["value"] / (total / cpu_s), 2)
 
 
+def mixed_leg(args):
+    """The mixed workload (configs n = 4, 6, 8, 12 in one launch) beside the main line: one
+    learned and one plain launch after a warm-up each, kernel time, no CPU baseline."""
+    sub = {}
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.no_cpu, a.layout = 1, 1, True, "lane"
+    bench_mixed(a, sub)
+    return {"workload": sub["config"]["workload"], "paths": sub["config"]["paths"],
+            "learned_paths_per_s": sub["value"], "learned_kernel_ms": sub["ms_per_step"],
+            "plain_paths_per_s": sub["plain_gpu"]["value"], "plain_kernel_ms": sub["plain_gpu"]["kernel_ms"],
+            "learned_over_plain": sub["learned"]["speedup_vs_plain"],
+            "identical_return_codes_learned_vs_plain": sub["identical_return_codes_learned_vs_plain"],
+            "roofline_frac_learned": sub["roofline"]["frac"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +528,7 @@ def main():
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
     ap.add_argument("--no-training", action="store_true", help="skip the proposer-training leg (param10)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay leg (param10)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-workload leg (param10)")
     ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
                     help="threads of the CPU baseline (default: every CPU this process may use)")
     ap.add_argument("--cpu-targets", type=int, default=192, help="targets of the CPU baseline sample (param10)")
@@ -521,8 +537,8 @@ def main():
                          "scaling FETCH_SIZE / WRITE_SIZE counter readings) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: paths are independent and this build has no multi-GPU path "
-              "(see DESIGN.md for how the batch would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: paths are independent, so a batch would shard by path index "
+              "(one persistent launch per GPU), and this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     import torch
     if args.copy_calibration_mib:
@@ -541,6 +557,8 @@ def main():
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}
     {"param10": bench_param10, "dense8": bench_dense8, "mixed": bench_mixed}[args.workload](args, out)
+    if args.workload == "param10" and not args.no_mixed:
+        out["mixed_workload"] = mixed_leg(args)
     print(json.dumps(out))
 
 

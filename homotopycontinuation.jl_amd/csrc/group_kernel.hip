@@ -17,7 +17,7 @@ constexpr int kBlock = 256;
 constexpr int kMlpLdsBytes = HCG_GROUP_MLP_LDS;
 
 template <int NN>
-__global__ void __launch_bounds__(kBlock, 1) hcg_group_kernel(hcg_multi_kargs m, const hcg_gsys* sysd, int lds_per_path) {
+__global__ void __launch_bounds__(kBlock, 2) hcg_group_kernel(hcg_multi_kargs m, const hcg_gsys* sysd, int lds_per_path) {
     extern __shared__ __align__(16) char smem[];
     hcg_group_run<NN>(m, sysd, lds_per_path, smem, (int)threadIdx.x);
 }
@@ -45,6 +45,21 @@ hipError_t group_launch(int nn, int grid, int lds_per_path, hipStream_t stream, 
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Workgroups of the kernel instance that fit on one CU with `shmem` bytes of dynamic LDS each
+// (registers and LDS both counted); each workgroup is one wave per SIMD.
+hipError_t group_kernel_blocks_per_cu(int nn, size_t shmem, int* blocks) {
+    hipError_t e = nn == 4 ? allow_lds<4>(shmem) : nn == 8 ? allow_lds<8>(shmem) : nn == 12 ? allow_lds<12>(shmem)
+                                                                              : allow_lds<16>(shmem);
+    if (e != hipSuccess) return e;
+    switch (nn) {
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, hcg_group_kernel<4>, kBlock, shmem);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, hcg_group_kernel<8>, kBlock, shmem);
+    case 12: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, hcg_group_kernel<12>, kBlock, shmem);
+    case 16: return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, hcg_group_kernel<16>, kBlock, shmem);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t group_kernel_attributes(int nn, hipFuncAttributes* attr) {

@@ -909,7 +909,8 @@ extern "C" int hcg_group_schedule_info(const hcg_homotopy_desc* const* hs, int32
 
 // Resource use of a program's kernel, from its code object: [0] VGPRs per lane (as the
 // loader reports them), [1] private segment bytes per lane, [2] static LDS bytes per
-// workgroup, [3] waves per SIMD the kernel was built for, [4] workgroup size, [5] CUs.
+// workgroup, [3] waves per SIMD the kernel was built for (lane-group layout: the
+// workgroups that fit on a CU, each one wave per SIMD), [4] workgroup size, [5] CUs.
 extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
     if (!p || !out6) return fail(HCG_ERR_INVALID_ARGUMENT, "NULL argument");
     if (p->group) {
@@ -919,7 +920,9 @@ extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
         out6[0] = fa.numRegs;
         out6[1] = (int32_t)fa.localSizeBytes;
         out6[2] = hcg::group_mlp_lds_bytes() + (hcg::group_block() / HCG_GROUP_LANES) * p->group->lds_per_path;
-        out6[3] = 1;
+        int blocks = 0;  // workgroups per CU; each holds one wave per SIMD
+        HIPCHK(hcg::group_kernel_blocks_per_cu(p->group->nn, (size_t)out6[2], &blocks));
+        out6[3] = blocks;
         out6[4] = hcg::group_block();
         out6[5] = p->ctx->ncu;
         return HCG_OK;
