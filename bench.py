@@ -464,6 +464,12 @@ def bench_mixed(args, out):
     pms, pfl, pouts = run(make_options())
     lms, lfl, louts = run(make_options(learned_step=1))
     same = all(bool((a.code == b.code).all()) for a, b in zip(pouts, louts))
+    jump = 0.0  # largest relative endpoint difference learned vs plain over paths both end in success
+    for a, b in zip(pouts, louts):
+        ok = (a.code == 1) & (b.code == 1)
+        if ok.any():
+            r = np.abs(b.solution[ok] - a.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(a.solution[ok]).max(axis=1))
+            jump = max(jump, float(r.max()))
     out.update({"value": round(total / lms * 1e3, 1), "ms_per_step": round(lms, 3),
                 "config": {"workload": "mixed: shared-factor systems n=4,6,8,12 (538812 total-degree paths) in one "
                                        "launch, learned-step tracker; value from the kernel time (inputs are copied "
@@ -474,6 +480,7 @@ def bench_mixed(args, out):
                 "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
                 "learned": {"speedup_vs_plain": round(pms / lms, 3)},
                 "identical_return_codes_learned_vs_plain": same,
+                "max_rel_endpoint_diff_learned_vs_plain": jump,
                 "solution_counts": {f"n{H.n}": counts(o.records) for (H, _), o in zip(probs, louts)}})
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
@@ -511,6 +518,7 @@ def mixed_leg(args):
             "plain_paths_per_s": sub["plain_gpu"]["value"], "plain_kernel_ms": sub["plain_gpu"]["kernel_ms"],
             "learned_over_plain": sub["learned"]["speedup_vs_plain"],
             "identical_return_codes_learned_vs_plain": sub["identical_return_codes_learned_vs_plain"],
+            "max_rel_endpoint_diff_learned_vs_plain": sub["max_rel_endpoint_diff_learned_vs_plain"],
             "roofline_frac_learned": sub["roofline"]["frac"]}
 
 

@@ -189,3 +189,59 @@ def combine_logs(parts):
         codes.append(np.asarray(c))
         off += len(c)
     return np.concatenate(logs), np.concatenate(codes)
+
+
+OFFSET_CANDIDATES = (0.0, -0.1, -0.2, -0.3, -0.4)
+
+
+def with_output_offset(params, kappa: float) -> np.ndarray:
+    """The network with κ subtracted from its log-step output (κ < 0: longer proposals)."""
+    p = np.array(params, dtype=np.float32, copy=True)
+    p[OFF_B3] -= np.float32(kappa)
+    return p
+
+
+def select_output_offset(ctx, params, runs, kappas=OFFSET_CANDIDATES, endpoint_rtol: float = 1e-8,
+                         slack: float = 0.05, repeats: int = 3):
+    """Model selection on the training systems: `runs` is a list of callables, each tracking
+    one training batch with the options it is given and returning (return codes, endpoints,
+    kernel ms).  Every κ is tried on every run.  A candidate is kept only if, on every training
+    path, its return code equals the plain tracker's and, for paths that end in success, its
+    endpoint equals the plain tracker's within `endpoint_rtol` (relative to max(1, |x|∞)):
+    a longer step must not let a path jump to another solution.  Of the kept candidates within
+    `slack` of the fastest (median over `repeats` passes of the total kernel time) the one
+    closest to κ = 0 wins: rare jumps that a training set of this size cannot show grow with the
+    step length.  Returns (weights with that offset, κ, table)."""
+    from .solve import make_options
+
+    def timed(opts):
+        """Results of every run and the median over `repeats` passes of their total kernel time."""
+        for run in runs:
+            run(opts)  # warm
+        passes = [[run(opts) for run in runs] for _ in range(repeats)]
+        return passes[0], float(np.median([sum(r[2] for r in p) for p in passes]))
+
+    plain, plain_ms = timed(make_options())
+    table = []
+    for k in kappas:
+        load(ctx, with_output_offset(params, k))
+        got, ms = timed(make_options(learned_step=1))
+        same = all(np.array_equal(g[0], pl[0]) for g, pl in zip(got, plain))
+        rel = 0.0
+        for g, pl in zip(got, plain):
+            ok = (pl[0] == 1) & (g[0] == 1)
+            if ok.any():
+                a, b = np.asarray(g[1])[ok], np.asarray(pl[1])[ok]
+                r = np.abs(a - b).max(axis=1) / np.maximum(1.0, np.abs(b).max(axis=1))
+                rel = max(rel, float(np.nan_to_num(r, nan=np.inf).max()))
+        keep = same and rel <= endpoint_rtol
+        table.append({"kappa": k, "learned_ms": round(ms, 3), "plain_ms": round(plain_ms, 3),
+                      "ratio": round(plain_ms / ms, 4), "identical_codes": bool(same),
+                      "max_rel_endpoint_diff": rel, "kept": bool(keep)})
+    kept = [r for r in table if r["kept"]]
+    if not kept:
+        raise RuntimeError("no output offset keeps the plain tracker's return codes and endpoints on the "
+                           "training systems")
+    fastest = min(r["learned_ms"] for r in kept)
+    best = min((r for r in kept if r["learned_ms"] <= fastest * (1 + slack)), key=lambda r: abs(r["kappa"]))["kappa"]
+    return with_output_offset(params, best), best, table

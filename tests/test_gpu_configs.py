@@ -83,6 +83,10 @@ def test_cfg2_full_batch_plain_and_learned(gpu_ctx):
     learned.load(gpu_ctx, learned.saved_weights("param10_step_mlp"))
     gl = track_batch(prog, starts, make_options(learned_step=1), group_of_path=groups)
     assert list(gl.code) == list(gp.code) and list(gl.singular) == list(gp.singular)
+    # no path jumps: every learned endpoint is the plain tracker's solution (all 262,144 paths)
+    ok = (gp.code == 1) & (gl.code == 1)
+    jump = np.abs(gl.solution[ok] - gp.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(gp.solution[ok]).max(axis=1))
+    assert jump.max() <= TOL, f"a learned path ended at another solution (rel {jump.max():.2e})"
     # the oracle on 2 of the 256 targets (2,048 paths): every path, every field that decides counts
     m = 2 * len(S)
     o = oracle.track_batch(H, starts[:m], make_options(), group_of_path=groups[:m])
@@ -113,9 +117,16 @@ def test_cfg4_mixed_learned_vs_oracle(gpu_ctx):
     multi = gpu_ctx.program_multi(Hs)
     learned.load(gpu_ctx, learned.saved_weights("mixed_step_mlp"))
     outs = track_multi(multi, starts, make_options(learned_step=1))
+    plain = track_multi(multi, starts, make_options())
     rng = np.random.default_rng(4)
     lines = []
-    for H, S, g in zip(Hs, starts, outs):
+    for H, S, g, gp in zip(Hs, starts, outs, plain):
+        # every path: the plain tracker's code, and no jump to another solution
+        assert list(g.code) == list(gp.code) and list(g.singular) == list(gp.singular)
+        ok = (g.code == 1) & (gp.code == 1)
+        if ok.any():
+            jump = np.abs(g.solution[ok] - gp.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(gp.solution[ok]).max(axis=1))
+            assert jump.max() <= TOL, f"n={H.n}: a learned path ended at another solution (rel {jump.max():.2e})"
         m = len(S) if H.n < 12 else 8192
         o = oracle.track_batch(H, S[:m], make_options())
         rel_o = _same_as_oracle(g.code[:m], g.singular[:m], g.solution[:m], o)

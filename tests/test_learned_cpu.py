@@ -71,6 +71,36 @@ def test_fold_scaling_preserves_the_function():
     np.testing.assert_allclose(o1, o2, rtol=1e-4, atol=1e-4)
 
 
+def test_output_offset_shifts_only_the_log_step_output():
+    """with_output_offset(p, κ) lowers the log-step output by exactly κ and leaves the
+    at-infinity output and every other parameter unchanged."""
+    X, _, _ = _data(40, 3)
+    p = learned.init_params(3)
+    q = learned.with_output_offset(p, -0.25)
+    o1, _ = mlp_ref.forward(p, X, bf16=False)
+    o2, _ = mlp_ref.forward(q, X, bf16=False)
+    np.testing.assert_allclose(o2[:, 0] - o1[:, 0], 0.25, rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(o2[:, 1], o1[:, 1])
+    diff = np.flatnonzero(p != q)
+    assert list(diff) == [learned.OFF_B3]
+
+
+@pytest.mark.parametrize("name", ["param10_step_mlp", "mixed_step_mlp"])
+def test_committed_offset_follows_the_selection_rule(name):
+    """The offset recorded beside each committed network is the one select_output_offset's rule
+    picks from the recorded table: kept (training codes and endpoints equal the plain
+    tracker's), within 5% of the fastest kept candidate, and closest to 0 among those."""
+    import json
+    from pathlib import Path
+    meta = json.loads((Path(learned.__file__).parent / "weights" / f"{name}.json").read_text())
+    kept = [r for r in meta["table"] if r["kept"]]
+    assert all(r["identical_codes"] and r["max_rel_endpoint_diff"] <= 1e-8 for r in kept)
+    fastest = min(r["learned_ms"] for r in kept)
+    pick = min((r for r in kept if r["learned_ms"] <= fastest * 1.05), key=lambda r: abs(r["kappa"]))
+    assert pick["kappa"] == meta["kappa"]
+    assert learned.saved_weights(name).shape == (learned.NPARAMS,)
+
+
 def test_training_targets_from_a_step_log():
     """Path 0: steps of size 0.1 (acc), 0.2 (rejected), 0.15 (acc), 0.3 (acc), 0.3 (acc).
     Target of step 0 = log((0.15 + 0.3) / plain_0); step 2 = log((0.3 + 0.3) / plain_2)."""
