@@ -179,6 +179,14 @@ def lib():
     path = os.environ.get("HCGPU_LIB", str(LIB_PATH))
     if not os.path.exists(path):
         raise HcgpuError(f"libhcgpu.so not built ({path}); run __graft_entry__.build()")
+    # One HIP runtime per process.  torch ships its own libamdhip64 / libhsa-runtime64 under the
+    # same sonames as /opt/rocm's: loaded first, it is the one libhcgpu.so binds to.  Loaded the
+    # other way round the process holds two runtimes, and torch then finds no GPU, so device
+    # buffers from torch (hcgpu.device) would fail.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     P = ctypes.POINTER
     L.hcg_ctx_create.argtypes = [ctypes.c_int, P(ctypes.c_void_p)]
