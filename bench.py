@@ -409,20 +409,35 @@ def bench_dense8(args, out):
     from hcgpu.gpu import GpuContext
     from hcgpu.homotopies import total_degree
     from hcgpu.solve import make_options
+    from hcgpu import autotune
     H, S = total_degree(dense_system(8, 3, seed=1), seed=2)
     ctx = GpuContext(0)
-    # 6,561 paths fill 103 waves, far fewer than the GPU's SIMDs: the kernel is built for one
-    # wave per SIMD (the largest register budget) instead of the default four
-    prog = ctx.program(H, waves_per_eu=1) if args.layout != "group" else ctx.program(H, layout="group")
+    opts = make_options()
+    if args.layout == "auto":
+        # 6,561 paths fill 103 waves of the per-lane kernel, far fewer than the GPU's 1,024
+        # SIMDs, or 1,641 waves of the lane-group kernel (4 paths per wave): the tuner times both
+        cands = autotune.candidates_for(H.n, lane=((1, 4), (1, 8), (2, 8)), group=(1, 2))
+        t0 = time.perf_counter()
+        tuned = autotune.tune(ctx, H, S, opts, candidates=cands, force=args.retune)
+        launch = {"chosen_by": "autotuner", "tuning_s": round(time.perf_counter() - t0, 1), "table": tuned["table"]}
+        prog = ctx.program(H)
+    else:
+        prog = ctx.program(H, waves_per_eu=1) if args.layout == "lane" else ctx.program(H, layout="group")
+        launch = {"chosen_by": "--layout"}
+    opts.blocks_per_cu = prog.blocks_per_cu
+    launch.update({"layout": prog.layout, "waves_per_simd_built": prog.kernel_info()["waves_per_simd_built"],
+                   "blocks_per_cu": prog.blocks_per_cu or None})
     batch = DeviceBatch(ctx, prog, S)
-    r = timed(batch, make_options(), args.steps, args.warmup)
+    r = timed(batch, opts, args.steps, args.warmup)
     rec = batch.results()[0]
     npaths = len(S)
     out.update({"value": round(npaths * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
                 "config": {"workload": "dense8: n=8, 8 dense cubics (165 monomials each), 6561 total-degree paths, "
-                                       "one persistent launch, plain tracker", "n": 8, "paths": npaths},
-                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths", info=prog.kernel_info(),
-                                     grid_waves=grid_waves(prog, npaths)),
+                                       "one persistent launch, plain tracker", "n": 8, "paths": npaths,
+                           "launch": launch},
+                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths" + ("" if prog.layout == "lane" else "_group"),
+                                     kernel="hcg_track_kernel" if prog.layout == "lane" else "hcg_group_kernel",
+                                     info=prog.kernel_info(), grid_waves=grid_waves(prog, npaths)),
                 "steps_per_path": round(r["steps"] / npaths, 3), "solution_counts": counts(rec)})
     if not args.no_cpu:
         sys.path.insert(0, str(REPO))
@@ -530,7 +545,7 @@ def main():
     ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
     ap.add_argument("--retune", action="store_true", help="tune again even when the autotuner has a cached choice")
     ap.add_argument("--layout", choices=["auto", "lane", "group"], default="auto",
-                    help="auto: the autotuner's choice (param10; the other workloads take lane); lane: one path "
+                    help="auto: the autotuner's choice (param10, dense8; the mixed workload takes lane); lane: one path "
                          "per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
