@@ -108,6 +108,7 @@ def tune(ctx: GpuContext, homotopy, starts, opts: Optional[abi.hcg_options] = No
     entry = {"layout": best["layout"], "waves_per_eu": best["waves_per_eu"], "blocks_per_cu": best["blocks_per_cu"],
              "sample_paths": len(starts), "table": table}
     _store(key, entry)
+    ctx._tuned[id(homotopy)] = entry  # later ctx.program(homotopy) calls build the tuned choice
     return entry
 
 
