@@ -179,14 +179,11 @@ def lib():
     path = os.environ.get("HCGPU_LIB", str(LIB_PATH))
     if not os.path.exists(path):
         raise HcgpuError(f"libhcgpu.so not built ({path}); run __graft_entry__.build()")
-    # One HIP runtime per process.  torch ships its own libamdhip64 / libhsa-runtime64 under the
-    # same sonames as /opt/rocm's: loaded first, it is the one libhcgpu.so binds to.  Loaded the
-    # other way round the process holds two runtimes, and torch then finds no GPU, so device
-    # buffers from torch (hcgpu.device) would fail.
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    # The library binds whichever HIP runtime and hipRTC the process already holds under their
+    # sonames (torch's bundled copies when torch was imported first, else /opt/rocm's).  It does
+    # not import torch itself: the code-object cache is compiled by __graft_entry__.build() with
+    # /opt/rocm's hipRTC, whose code is faster than torch's older bundled compiler's.  Code that
+    # uses torch device buffers (hcgpu.device) imports torch before this library is loaded.
     L = ctypes.CDLL(path)
     P = ctypes.POINTER
     L.hcg_ctx_create.argtypes = [ctypes.c_int, P(ctypes.c_void_p)]

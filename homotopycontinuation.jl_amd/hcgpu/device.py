@@ -7,6 +7,10 @@ import ctypes
 from typing import Optional
 
 import numpy as np
+# torch before libhcgpu.so: torch bundles its own HIP runtime under the sonames of /opt/rocm's;
+# imported first, it is the runtime the library binds to as well.  The other way round the
+# process holds two runtimes and torch finds no GPU.
+import torch
 
 from . import abi
 from .abi import check, hcg_batch, hcg_options, hcg_path_result
@@ -19,9 +23,13 @@ class DeviceBatch:
     """A batch whose starts, parameters, group indices and outputs live in device memory."""
 
     def __init__(self, ctx, prog, starts, p=None, Q=None, groups=None, device: int = 0):
-        import torch
         self._torch = torch
         dev = torch.device(f"cuda:{device}")
+        try:
+            torch.cuda.init()
+        except RuntimeError as e:
+            raise abi.HcgpuError("torch cannot reach the GPU: import torch (or hcgpu.device) before the first "
+                                 "GpuContext, so that torch and libhcgpu.so share one HIP runtime") from e
         starts = np.ascontiguousarray(starts, dtype=np.complex128)
         self.npaths, self.n = starts.shape
         self.ctx, self.prog = ctx, prog

@@ -146,6 +146,19 @@ def test_graph_entry_points_reject_missing_arguments():
     L.hcg_graph_destroy(None)
 
 
+def test_loading_the_library_does_not_import_torch():
+    """The code-object cache is compiled in processes that load libhcgpu.so without torch, so
+    that /opt/rocm's hipRTC compiles it (torch bundles an older hipRTC under the same soname,
+    whose code ran the parameter workload 27% slower)."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import hcgpu; from hcgpu import abi; abi.lib(); "
+            "print('torch' in sys.modules)" % str(REPO / "homotopycontinuation.jl_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "False"
+
+
 def test_no_gpu_means_a_loud_failure():
     """The product path has no CPU fallback: without a device the context cannot be made."""
     import torch

@@ -7,6 +7,7 @@ from pathlib import Path
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (before libhcgpu.so: one HIP runtime)
 
 from hcgpu.abi import HcgpuError, RETURN_CODES
 from hcgpu.configs import cfg2_problem, dense_system
