@@ -9,8 +9,10 @@ with the endgame.  Timed: the learned-step tracker (`value`) and the plain track
 with inputs resident in HBM (hcg_track_batch_device).  Beside them: the CPU oracle (a C
 restatement of the reference tracker) on a bounded sample of the same batch.
 
-Other workloads (--workload): "dense8" = 8 dense cubics in 8 unknowns, 6,561 total-degree
-paths; "mixed" = shared-factor systems of n = 4, 6, 8, 12 (538,812 paths) in one launch.
+Other workloads (--workload): "dense4" = 4 dense cubics in 4 unknowns, 81 total-degree paths;
+"dense8" = 8 dense cubics in 8 unknowns, 6,561 total-degree paths; "mixed" = shared-factor
+systems of n = 4, 6, 8, 12 (538,812 paths) in one launch.  The default line also carries the
+mixed workload's learned and plain launches (`mixed_workload`).
 
 The JSON line carries the counted-FLOP roofline of the timed kernel (HIP events on the
 launch stream) and, when profiled, its HBM traffic from profiles/traffic.json.
@@ -404,18 +406,24 @@ def bench_param10(args, out):
             "max_rel_endpoint_diff_learned": float(relc.max()) if okc.any() else 0.0}
 
 
-def bench_dense8(args, out):
+# dense total-degree workloads: (n, system seed, gamma seed), the sizes and seeds of the GPU tests
+DENSE = {"dense4": (4, 100, 101), "dense8": (8, 1, 2)}
+
+
+def bench_dense(args, out):
     from hcgpu.configs import dense_system
     from hcgpu.gpu import GpuContext
     from hcgpu.homotopies import total_degree
     from hcgpu.solve import make_options
     from hcgpu import autotune
-    H, S = total_degree(dense_system(8, 3, seed=1), seed=2)
+    n, sseed, gseed = DENSE[args.workload]
+    H, S = total_degree(dense_system(n, 3, seed=sseed), seed=gseed)
     ctx = GpuContext(0)
     opts = make_options()
     if args.layout == "auto":
-        # 6,561 paths fill 103 waves of the per-lane kernel, far fewer than the GPU's 1,024
-        # SIMDs, or 1,641 waves of the lane-group kernel (4 paths per wave): the tuner times both
+        # 3^n paths (81 or 6,561) fill 2 or 103 waves of the per-lane kernel, far fewer than the
+        # GPU's 1,024 SIMDs, or 21 or 1,641 waves of the lane-group kernel (4 paths per wave):
+        # the tuner times both
         cands = autotune.candidates_for(H.n, lane=((1, 4), (1, 8), (2, 8)), group=(1, 2))
         t0 = time.perf_counter()
         tuned = autotune.tune(ctx, H, S, opts, candidates=cands, force=args.retune)
@@ -432,10 +440,10 @@ def bench_dense8(args, out):
     rec = batch.results()[0]
     npaths = len(S)
     out.update({"value": round(npaths * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
-                "config": {"workload": "dense8: n=8, 8 dense cubics (165 monomials each), 6561 total-degree paths, "
-                                       "one persistent launch, plain tracker", "n": 8, "paths": npaths,
+                "config": {"workload": f"{args.workload}: n={n}, {n} dense cubics ({len(S)} total-degree paths), "
+                                       "one persistent launch, plain tracker", "n": n, "paths": npaths,
                            "launch": launch},
-                "roofline": roofline(r["flops"], r["kernel_ms"], "dense_n8_6561paths" + ("" if prog.layout == "lane" else "_group"),
+                "roofline": roofline(r["flops"], r["kernel_ms"], f"dense_n{n}_{npaths}paths" + ("" if prog.layout == "lane" else "_group"),
                                      kernel="hcg_track_kernel" if prog.layout == "lane" else "hcg_group_kernel",
                                      info=prog.kernel_info(), grid_waves=grid_waves(prog, npaths)),
                 "steps_per_path": round(r["steps"] / npaths, 3), "solution_counts": counts(rec)})
@@ -444,7 +452,7 @@ def bench_dense8(args, out):
         import oracle  # CPU baseline only
         from hcgpu.tape import shared_lowering
         with shared_lowering():  # the CPU evaluates tapes with every subexpression shared
-            Hc, _ = total_degree(dense_system(8, 3, seed=1), seed=2)
+            Hc, _ = total_degree(dense_system(n, 3, seed=sseed), seed=gseed)
         t0 = time.perf_counter()
         o = oracle.track_batch(Hc, S, make_options(), nthreads=args.cpu_threads)
         dt = time.perf_counter() - t0
@@ -493,7 +501,13 @@ def bench_mixed(args, out):
                 "roofline": roofline(lfl, lms, "mixed_learned", kernel="hcg_track_multi_kernel",
                                      info=prog.kernel_info(), grid_waves=grid_waves(prog, total)),
                 "plain_gpu": {"value": round(total / pms * 1e3, 1), "unit": "paths/s", "kernel_ms": round(pms, 3)},
-                "learned": {"speedup_vs_plain": round(pms / lms, 3)},
+                "learned": {"speedup_vs_plain": round(pms / lms, 3),
+                            "steps_per_path": round(sum(int((o.accepted_steps + o.rejected_steps).sum()) for o in louts)
+                                                    / total, 3),
+                            "acceptance_rate": round(sum(int(o.learned_accepted.sum()) for o in louts)
+                                                     / max(1, sum(int(o.learned_proposed.sum()) for o in louts)), 4)},
+                "plain_steps_per_path": round(sum(int((o.accepted_steps + o.rejected_steps).sum()) for o in pouts)
+                                              / total, 3),
                 "identical_return_codes_learned_vs_plain": same,
                 "max_rel_endpoint_diff_learned_vs_plain": jump,
                 "solution_counts": {f"n{H.n}": counts(o.records) for (H, _), o in zip(probs, louts)}})
@@ -532,6 +546,9 @@ def mixed_leg(args):
             "learned_paths_per_s": sub["value"], "learned_kernel_ms": sub["ms_per_step"],
             "plain_paths_per_s": sub["plain_gpu"]["value"], "plain_kernel_ms": sub["plain_gpu"]["kernel_ms"],
             "learned_over_plain": sub["learned"]["speedup_vs_plain"],
+            "learned_steps_per_path": sub["learned"]["steps_per_path"],
+            "plain_steps_per_path": sub["plain_steps_per_path"],
+            "learned_acceptance_rate": sub["learned"]["acceptance_rate"],
             "identical_return_codes_learned_vs_plain": sub["identical_return_codes_learned_vs_plain"],
             "max_rel_endpoint_diff_learned_vs_plain": sub["max_rel_endpoint_diff_learned_vs_plain"],
             "roofline_frac_learned": sub["roofline"]["frac"]}
@@ -542,10 +559,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["param10", "dense8", "mixed"], default="param10")
+    ap.add_argument("--workload", choices=["param10", "dense4", "dense8", "mixed"], default="param10")
     ap.add_argument("--retune", action="store_true", help="tune again even when the autotuner has a cached choice")
     ap.add_argument("--layout", choices=["auto", "lane", "group"], default="auto",
-                    help="auto: the autotuner's choice (param10, dense8; the mixed workload takes lane); lane: one path "
+                    help="auto: the autotuner's choice (param10, dense4, dense8; the mixed workload takes lane); lane: one path "
                          "per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
@@ -579,7 +596,7 @@ def main():
     out = {"metric": METRIC, "value": None, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}
-    {"param10": bench_param10, "dense8": bench_dense8, "mixed": bench_mixed}[args.workload](args, out)
+    {"param10": bench_param10, "dense4": bench_dense, "dense8": bench_dense, "mixed": bench_mixed}[args.workload](args, out)
     if args.workload == "param10" and not args.no_mixed:
         out["mixed_workload"] = mixed_leg(args)
     print(json.dumps(out))
