@@ -11,7 +11,8 @@ restatement of the reference tracker) on a bounded sample of the same batch.
 
 Other workloads (--workload): "dense4" = 4 dense cubics in 4 unknowns, 81 total-degree paths;
 "dense8" = 8 dense cubics in 8 unknowns, 6,561 total-degree paths; "mixed" = shared-factor
-systems of n = 4, 6, 8, 12 (538,812 paths) in one launch.  The default line also carries the
+systems of n = 4, 6, 8, 12 (538,812 paths) in one launch; "steiner" = the Steiner conics
+(n = 15) tracked to 8,192 nearby targets.  The default line also carries the
 mixed workload's learned and plain launches (`mixed_workload`).
 
 The JSON line carries the counted-FLOP roofline of the timed kernel (HIP events on the
@@ -461,6 +462,74 @@ def bench_dense(args, out):
         out["parity_vs_oracle"] = {"identical_return_codes": bool((o.code == rec["code"]).all())}
 
 
+def steiner_problem(m, seed=5):
+    """Steiner conics (n = 15, 30 parameters; test/test_cases/steiner_higher_prec.jl): the
+    reference's start solution s_p at p tracked to m targets near its q (every coefficient
+    perturbed by 1e-3 relative, seeded)."""
+    import numpy as np
+    sys.path.insert(0, str(REPO / "tests"))
+    import systems
+    F, p, q, s_p, _ = systems.steiner()
+    rng = np.random.default_rng(seed)
+    Q = np.asarray(q)[None, :] * (1 + 1e-3 * (rng.standard_normal((m, len(q))) + 1j * rng.standard_normal((m, len(q)))))
+    return F, np.asarray(p), Q, np.asarray(s_p)
+
+
+def bench_steiner(args, out):
+    import numpy as np
+    from hcgpu import autotune
+    from hcgpu.gpu import GpuContext
+    from hcgpu.homotopies import ParameterHomotopy
+    from hcgpu.solve import make_options
+    m = 8192
+    F, p, Q, s_p = steiner_problem(m)
+    H = ParameterHomotopy(F, p, Q)
+    S = np.tile(s_p[None, :], (m, 1))
+    groups = np.arange(m, dtype=np.int32)
+    ctx = GpuContext(0)
+    opts = make_options()
+    if args.layout == "auto":
+        # n = 15 is past the per-lane kernel's register-resident LU (n <= 12): the tuner times the
+        # lane-group kernel beside it
+        cands = autotune.candidates_for(H.n, lane=((1, 4), (1, 8), (2, 8)), group=(1, 2))
+        t0 = time.perf_counter()
+        tuned = autotune.tune(ctx, H, S, opts, groups, candidates=cands, force=args.retune)
+        launch = {"chosen_by": "autotuner", "tuning_s": round(time.perf_counter() - t0, 1), "table": tuned["table"]}
+        prog = ctx.program(H)
+    else:
+        prog = ctx.program(H, layout=args.layout)
+        launch = {"chosen_by": "--layout"}
+    opts.blocks_per_cu = prog.blocks_per_cu
+    launch.update({"layout": prog.layout, "waves_per_simd_built": prog.kernel_info()["waves_per_simd_built"],
+                   "blocks_per_cu": prog.blocks_per_cu or None})
+    batch = DeviceBatch(ctx, prog, S, H.p, H.q, groups)
+    r = timed(batch, opts, args.steps, args.warmup)
+    rec, sol = batch.results()[:2]
+    out.update({"value": round(m * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
+                "config": {"workload": f"steiner: n=15, 30 parameters, the reference's Steiner start tracked to {m} "
+                                       "targets near its q in one persistent launch, plain tracker", "n": 15,
+                           "paths": m, "launch": launch},
+                "roofline": roofline(r["flops"], r["kernel_ms"], f"steiner_{m}paths" + ("" if prog.layout == "lane" else "_group"),
+                                     kernel="hcg_track_kernel" if prog.layout == "lane" else "hcg_group_kernel",
+                                     info=prog.kernel_info(), grid_waves=grid_waves(prog, m)),
+                "steps_per_path": round(r["steps"] / m, 3), "solution_counts": counts(rec)})
+    if not args.no_cpu:
+        sys.path.insert(0, str(REPO))
+        import oracle  # CPU baseline only
+        k = 512
+        t0 = time.perf_counter()
+        o = oracle.track_batch(H, S[:k], opts, nthreads=args.cpu_threads, group_of_path=groups[:k])
+        dt = time.perf_counter() - t0
+        ok = (o.code == 1) & (rec["code"][:k] == 1)
+        rel = np.abs(sol[:k][ok] - o.solution[ok]).max(axis=1) / np.maximum(1.0, np.abs(o.solution[ok]).max(axis=1))
+        out["cpu_baseline"] = {"value": round(k / dt, 1), "unit": "paths/s", "cores": args.cpu_threads, "kind": "port",
+                               "build": args.oracle_build, "host": host_info(),
+                               "sample": f"first {k} of the {m} paths, oracle, {dt:.1f} s"}
+        out["gpu_vs_cpu"] = round(out["value"] / (k / dt), 2)
+        out["parity_vs_oracle_sample"] = {"identical_return_codes": bool((o.code == rec["code"][:k]).all()),
+                                          "max_rel_endpoint_diff": float(rel.max()) if ok.any() else 0.0}
+
+
 def bench_mixed(args, out):
     import numpy as np
     from hcgpu import learned
@@ -559,10 +628,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["param10", "dense4", "dense8", "mixed"], default="param10")
+    ap.add_argument("--workload", choices=["param10", "dense4", "dense8", "mixed", "steiner"], default="param10")
     ap.add_argument("--retune", action="store_true", help="tune again even when the autotuner has a cached choice")
     ap.add_argument("--layout", choices=["auto", "lane", "group"], default="auto",
-                    help="auto: the autotuner's choice (param10, dense4, dense8; the mixed workload takes lane); lane: one path "
+                    help="auto: the autotuner's choice (param10, dense4, dense8, steiner; the mixed workload takes lane); lane: one path "
                          "per lane (hipRTC-compiled tapes); group: one path per 16 lanes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-certify", action="store_true", help="skip the certification leg (param10)")
@@ -596,7 +665,8 @@ def main():
     out = {"metric": METRIC, "value": None, "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}
-    {"param10": bench_param10, "dense4": bench_dense, "dense8": bench_dense, "mixed": bench_mixed}[args.workload](args, out)
+    {"param10": bench_param10, "dense4": bench_dense, "dense8": bench_dense, "mixed": bench_mixed,
+     "steiner": bench_steiner}[args.workload](args, out)
     if args.workload == "param10" and not args.no_mixed:
         out["mixed_workload"] = mixed_leg(args)
     print(json.dumps(out))

@@ -92,6 +92,28 @@ def test_group_steiner_and_invalid_starts(gpu_ctx):
     assert g.codes() == ["terminated_invalid_startvalue_singular_jacobian"]
 
 
+def test_group_steiner_batch_equals_lane_and_oracle(gpu_ctx):
+    """n = 15, past the per-lane kernel's register-resident LU: the Steiner start tracked to 512
+    nearby targets (the bench.py steiner workload's first 512) in both layouts.  Codes, step
+    counts and endpoints are equal bit for bit between the layouts and to the oracle on a
+    sample."""
+    rng = np.random.default_rng(5)
+    F, p, q, s_p, _ = systems.steiner()
+    m = 512
+    Q = np.asarray(q)[None, :] * (1 + 1e-3 * (rng.standard_normal((m, len(q))) + 1j * rng.standard_normal((m, len(q)))))
+    H = ParameterHomotopy(F, p, Q)
+    S = np.tile(np.asarray(s_p)[None, :], (m, 1))
+    groups = np.arange(m, dtype=np.int32)
+    g = track_batch(gpu_ctx.program(H, layout="group"), S, make_options(), group_of_path=groups)
+    ln = track_batch(gpu_ctx.program(H, layout="lane"), S, make_options(), group_of_path=groups)
+    assert (g.code == 1).all()
+    assert_same(g, ln, "steiner batch, group vs lane")
+    k = 32
+    o = oracle.track_batch(H, S[:k], make_options(), group_of_path=groups[:k])
+    sub = track_batch(gpu_ctx.program(H, layout="group"), S[:k], make_options(), group_of_path=groups[:k])
+    assert_same(sub, o, "steiner batch sample, group vs oracle")
+
+
 def test_group_mixed_sizes_one_launch(gpu_ctx):
     probs = cfg4_problem()
     Hs = [H for H, _ in probs]
