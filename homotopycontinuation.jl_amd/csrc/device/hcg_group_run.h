@@ -179,7 +179,8 @@ __device__ __forceinline__ void hcg_group_run(const hcg_multi_kargs& m, const hc
     const unsigned short* mw = stage_mlp(m.sys[0].mlp, opts, (unsigned short*)smem);
     GT<NN> T;
     T.r = g_lane();
-    T.lds = smem + HCG_GROUP_MLP_LDS + (tid >> 4) * lds_per_path;
+    // the step network's weights take the front of LDS only when the learned step is on
+    T.lds = smem + (mw ? HCG_GROUP_MLP_LDS : 0) + (tid >> 4) * lds_per_path;
     T.mlp_w = mw;
     T.o = opts;
     T.cond_H_xd = 1.0;

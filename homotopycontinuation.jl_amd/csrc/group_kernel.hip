@@ -31,9 +31,10 @@ template <int NN> hipError_t allow_lds(size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t group_launch(int nn, int grid, int lds_per_path, hipStream_t stream, const hcg_multi_kargs& m,
-                        const hcg_gsys* d_sys) {
-    const size_t shmem = (size_t)kMlpLdsBytes + (size_t)(kBlock / HCG_GROUP_LANES) * lds_per_path;
+hipError_t group_launch(int nn, int grid, int lds_per_path, bool mlp_lds, hipStream_t stream,
+                        const hcg_multi_kargs& m, const hcg_gsys* d_sys) {
+    // LDS: the step network's BF16 weights (learned step only), then each group's registers
+    const size_t shmem = (mlp_lds ? (size_t)kMlpLdsBytes : 0) + (size_t)(kBlock / HCG_GROUP_LANES) * lds_per_path;
     hipError_t e = nn == 4 ? allow_lds<4>(shmem) : nn == 8 ? allow_lds<8>(shmem) : nn == 12 ? allow_lds<12>(shmem)
                                                                               : allow_lds<16>(shmem);
     if (e != hipSuccess) return e;

@@ -192,7 +192,9 @@ bool schedule_tapes(const std::vector<const hcg_tape*>& tapes, const std::vector
         for (auto& o : L.u[t]) use(o, END);
         for (auto& o : L.U[t]) use(o, END);
     }
-    // linear scan: a register becomes free after the last slot that reads it
+    // linear scan: a register read for the last time in a slot may take that slot's results,
+    // because the interpreter reads every operand of a slot before it writes any result (the
+    // 16 lanes of a group are one wave's instruction stream); check_program verifies it
     std::vector<int> reg_i(N, -1), reg_v(L.nvarvals, -1);
     std::set<int> freeregs;
     int nregs = 0;
@@ -205,7 +207,7 @@ bool schedule_tapes(const std::vector<const hcg_tape*>& tapes, const std::vector
         if (last_v[v] >= 0) { reg_v[v] = alloc(); release.insert({last_v[v], reg_v[v]}); }
     for (int si = 0; si < S; si++) {
         // values whose last read is before this slot free their registers
-        while (!release.empty() && release.begin()->first < si) {
+        while (!release.empty() && release.begin()->first <= si) {
             freeregs.insert(release.begin()->second);
             release.erase(release.begin());
         }

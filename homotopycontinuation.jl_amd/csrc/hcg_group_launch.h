@@ -6,8 +6,10 @@
 
 namespace hcg {
 
-hipError_t group_launch(int nn, int grid, int lds_per_path, hipStream_t stream, const hcg_multi_kargs& m,
-                        const hcg_gsys* d_sys);
+// mlp_lds: stage the step network's weights in LDS (when the learned step is on); the kernel
+// stages them exactly when the launch's options ask for the learned step and weights are given.
+hipError_t group_launch(int nn, int grid, int lds_per_path, bool mlp_lds, hipStream_t stream,
+                        const hcg_multi_kargs& m, const hcg_gsys* d_sys);
 hipError_t group_kernel_attributes(int nn, hipFuncAttributes* attr);
 hipError_t group_kernel_blocks_per_cu(int nn, size_t shmem, int* blocks);
 

@@ -472,7 +472,8 @@ static int group_launch(hcg_ctx* ctx, const hcg_program* prog, const hcg_multi_k
     const int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 2;
     long grid = std::min<long>((total + per_block - 1) / per_block, (long)ctx->ncu * per_cu);
     if (grid < 1) grid = 1;
-    hipError_t e = hcg::group_launch(prog->group->nn, (int)grid, prog->group->lds_per_path, stream, m,
+    const bool mlp_lds = opts->learned_step && m.sys[0].mlp;  // the kernel's own staging condition
+    hipError_t e = hcg::group_launch(prog->group->nn, (int)grid, prog->group->lds_per_path, mlp_lds, stream, m,
                                      prog->group->d_sys);
     if (e != hipSuccess) return fail(HCG_ERR_HIP, "group kernel launch: %s", hipGetErrorString(e));
     return HCG_OK;
