@@ -23,6 +23,26 @@ __device__ __forceinline__ int g_int_imm(int32_t ref) { return ((int)((unsigned)
 // this marks the slot boundary for the compiler, so no LDS access moves across it.
 __device__ __forceinline__ void g_slot_end() { __builtin_amdgcn_wave_barrier(); }
 
+// One slot's table entries for lane r.  The interpreter loads slot s+1's entries before it runs
+// slot s, so their memory latency overlaps the slot's operand fetches and arithmetic.
+struct GSlot {
+    int op, o;
+    int4 a;
+};
+__device__ __forceinline__ GSlot g_slot(const hcg_gprog* P, int s, int r) {
+    GSlot q;
+    if (s < P->nslots) {
+        q.op = P->op[s];
+        q.a = ((const int4*)P->arg)[s * HCG_GROUP_LANES + r];
+        q.o = P->out[s * HCG_GROUP_LANES + r];
+    } else {
+        q.op = 0;
+        q.a = make_int4(0, 0, 0, 0);
+        q.o = -1;
+    }
+    return q;
+}
+
 // ------------------------------- ComplexF64 -------------------------------
 __device__ __forceinline__ cplx g_fetch_c(const GEval& E, int32_t ref) {
     const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
@@ -67,11 +87,12 @@ __device__ __forceinline__ void g_run_c(GEval E, int r, int n, cplx x) {
             if (vr >= 0) R[vr] = x;
         }
     g_slot_end();
-    const int4* A = (const int4*)P->arg;
+    GSlot next = g_slot(P, 0, r);
     for (int s = 0; s < P->nslots; s++) {
-        const int op = P->op[s];
-        const int4 a = A[s * HCG_GROUP_LANES + r];
-        const int o = P->out[s * HCG_GROUP_LANES + r];
+        const GSlot cur = next;
+        next = g_slot(P, s + 1, r);
+        const int op = cur.op, o = cur.o;
+        const int4 a = cur.a;
         if (o >= 0) {
             cplx va = g_fetch_c(E, a.x);
             cplx vb = op == 22 ? mkc(0.0, 0.0) : g_fetch_c(E, a.y);
@@ -125,11 +146,12 @@ __device__ __forceinline__ void g_run_dd(GEval E, int r, int n, cplx x) {
             if (vr >= 0) R[vr] = cdd_from(x);
         }
     g_slot_end();
-    const int4* A = (const int4*)P->arg;
+    GSlot next = g_slot(P, 0, r);
     for (int s = 0; s < P->nslots; s++) {
-        const int op = P->op[s];
-        const int4 a = A[s * HCG_GROUP_LANES + r];
-        const int o = P->out[s * HCG_GROUP_LANES + r];
+        const GSlot cur = next;
+        next = g_slot(P, s + 1, r);
+        const int op = cur.op, o = cur.o;
+        const int4 a = cur.a;
         if (o >= 0) {
             cdd va = g_fetch_dd(E, a.x);
             cdd vb = op == 22 ? cdd_from(mkc(0.0, 0.0)) : g_fetch_dd(E, a.y);
@@ -193,11 +215,12 @@ template <int K> __device__ __forceinline__ void g_run_s(GEval E, int r, int n, 
         }
     }
     g_slot_end();
-    const int4* A = (const int4*)P->arg;
+    GSlot next = g_slot(P, 0, r);
     for (int s = 0; s < P->nslots; s++) {
-        const int op = P->op[s];
-        const int4 a = A[s * HCG_GROUP_LANES + r];
-        const int o = P->out[s * HCG_GROUP_LANES + r];
+        const GSlot cur = next;
+        next = g_slot(P, s + 1, r);
+        const int op = cur.op, o = cur.o;
+        const int4 a = cur.a;
         if (o >= 0) {
             ser<K> va = g_fetch_s<K>(E, a.x);
             ser<K> vb = op == 22 ? s_const<K>(mkc(0.0, 0.0)) : g_fetch_s<K>(E, a.y);
