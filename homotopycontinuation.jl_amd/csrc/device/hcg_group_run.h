@@ -109,6 +109,25 @@ __device__ __forceinline__ void g_do_eval(GT<NN>& T) {
     T.req = GREQ_NONE;
 }
 
+// The request type a wave serves in this round of its path loop.  The four groups of a wave
+// run a slot program together only when their requests agree; groups with another pending
+// request wait for a later round.  Corrector requests come first and the predictor's Taylor
+// requests last, so a group that has finished its step waits at the next step's first Taylor
+// request until the other groups of its wave arrive there, and from then on the groups step
+// together.  Every round serves at least one group (the type is one that some group is
+// waiting with), and a waiting group waits at most until each other group of its wave reaches
+// its next Taylor request or ends its path.  Only the order of evaluations across groups
+// changes, never a path's own sequence, so results do not change.
+__device__ __forceinline__ int g_wave_pick(int req) {
+    if (__ballot(req == GREQ_COND)) return GREQ_COND;
+    if (__ballot(req == GREQ_DD)) return GREQ_DD;
+    if (__ballot(req == GREQ_E)) return GREQ_E;
+    if (__ballot(req == GREQ_EJ)) return GREQ_EJ;
+    if (__ballot(req == GREQ_T3)) return GREQ_T3;
+    if (__ballot(req == GREQ_T2)) return GREQ_T2;
+    return GREQ_T1;
+}
+
 // PathResult(endgame_tracker, ...), src/endgame_tracker.jl:847-888 (residual from the last
 // evaluation, which path_run made at the reported point)
 template <int NN>
@@ -227,7 +246,11 @@ __device__ __forceinline__ void hcg_group_run(const hcg_multi_kargs& m, const hc
             T.co_addsmp = T.co_sw = T.co_ainf = 0;
             evals = 0;
         }
-        if (T.path_run(m.sys[s].t1)) {
+        // a request the wave did not serve in the last round is still pending
+        const bool live = T.req != GREQ_NONE || T.path_run(m.sys[s].t1);
+        if (live) {
+            const int pick = g_wave_pick(T.req);
+            if (T.req != GREQ_NONE && T.req != pick) continue;
             g_do_eval<NN>(T);
             // a hard bound on the evaluations of one path (far above what the step limits
             // allow), so that every group reaches the end of its loop whatever happens
