@@ -12,10 +12,70 @@
 struct GEval {
     const hcg_gprog* P;
     char* lds;           // this group's register file
+    const double* consts;  // P->consts (g_bind)
     cplx t;              // parameters are formed at t (parameter homotopies)
     const double* p;     // start parameters (or null)
     const double* q;     // target parameters of the path's group (or null)
 };
+
+// The register file is addressed through local (address space 3) pointers: ds_read/ds_write.
+// Through the generic pointer the accesses compile to flat instructions, whose waits also
+// cover every global load in flight (among them the next slot's table entries).
+typedef __attribute__((address_space(3))) double g_lds_d;
+__device__ __forceinline__ cplx g_ld_c(const char* R, int k) {
+    const g_lds_d* d = (const g_lds_d*)R + 2 * k;
+    return mkc(d[0], d[1]);
+}
+__device__ __forceinline__ void g_st_c(char* R, int k, cplx v) {
+    g_lds_d* d = (g_lds_d*)R + 2 * k;
+    d[0] = v.re;
+    d[1] = v.im;
+}
+__device__ __forceinline__ cdd g_ld_dd(const char* R, int k) {
+    const g_lds_d* d = (const g_lds_d*)R + 4 * k;
+    return mkcdd(mkdd(d[0], d[1]), mkdd(d[2], d[3]));
+}
+__device__ __forceinline__ void g_st_dd(char* R, int k, cdd v) {
+    g_lds_d* d = (g_lds_d*)R + 4 * k;
+    d[0] = v.re.hi;
+    d[1] = v.re.lo;
+    d[2] = v.im.hi;
+    d[3] = v.im.lo;
+}
+template <int K> __device__ __forceinline__ ser<K> g_ld_s(const char* R, int k) {
+    const g_lds_d* d = (const g_lds_d*)R + 2 * (K + 1) * k;
+    ser<K> v;
+#pragma unroll
+    for (int j = 0; j <= K; j++) v.c[j] = mkc(d[2 * j], d[2 * j + 1]);
+    return v;
+}
+template <int K> __device__ __forceinline__ void g_st_s(char* R, int k, const ser<K>& v) {
+    g_lds_d* d = (g_lds_d*)R + 2 * (K + 1) * k;
+#pragma unroll
+    for (int j = 0; j <= K; j++) {
+        d[2 * j] = v.c[j].re;
+        d[2 * j + 1] = v.c[j].im;
+    }
+}
+
+// Slot tables, constants and parameters are read through global (address space 1) pointers:
+// global loads count only against the vector-memory counter, so waiting for an LDS operand
+// does not also wait for them.
+typedef __attribute__((address_space(1))) const double g_glob_d;
+typedef __attribute__((address_space(1))) const int32_t g_glob_i;
+typedef __attribute__((address_space(1))) const uint8_t g_glob_u8;
+__device__ __forceinline__ cplx g_gl_c(const double* base, int k) {
+    const g_glob_d* d = (const g_glob_d*)base + 2 * k;
+    return mkc(d[0], d[1]);
+}
+__device__ __forceinline__ int g_gl_i(const int32_t* base, int k) { return ((const g_glob_i*)base)[k]; }
+// par_lerp (hcg_arith.h) on the values p_k, q_k: the same operations in the same order
+__device__ __forceinline__ cplx g_par_lerp(cplx t, cplx pk, cplx qk) {
+    cplx rr = c_add(c_scale(t.re, pk), c_scale(1.0 - t.re, qk));
+    cplx rc = c_add(c_mul(t, pk), c_mul(mkc(1.0 - t.re, -t.im), qk));
+    const bool real_t = t.im == 0.0;
+    return mkc(real_t ? rr.re : rc.re, real_t ? rr.im : rc.im);
+}
 
 __device__ __forceinline__ int g_int_imm(int32_t ref) { return ((int)((unsigned)ref << 4)) >> 4; }
 
@@ -25,16 +85,34 @@ __device__ __forceinline__ void g_slot_end() { __builtin_amdgcn_wave_barrier(); 
 
 // One slot's table entries for lane r.  The interpreter loads slot s+1's entries before it runs
 // slot s, so their memory latency overlaps the slot's operand fetches and arithmetic.
+// The table pointers are read from the program record once per run: the record is reached
+// through a generic pointer, which the compiler cannot tell apart from the LDS the slots write,
+// so it would otherwise read them again at every slot.
 struct GSlot {
     int op, o;
     int4 a;
 };
-__device__ __forceinline__ GSlot g_slot(const hcg_gprog* P, int s, int r) {
+struct GTables {
+    const uint8_t* op;
+    const int32_t* arg;
+    const int32_t* out;
+    int nslots;
+};
+__device__ __forceinline__ GTables g_tables(const hcg_gprog* P) {
+    GTables t;
+    t.op = P->op;
+    t.arg = P->arg;
+    t.out = P->out;
+    t.nslots = P->nslots;
+    return t;
+}
+__device__ __forceinline__ GSlot g_slot(const GTables& P, int s, int r) {
     GSlot q;
-    if (s < P->nslots) {
-        q.op = P->op[s];
-        q.a = ((const int4*)P->arg)[s * HCG_GROUP_LANES + r];
-        q.o = P->out[s * HCG_GROUP_LANES + r];
+    if (s < P.nslots) {
+        q.op = ((const g_glob_u8*)P.op)[s];
+        const g_glob_i* ap = (const g_glob_i*)P.arg + 4 * (s * HCG_GROUP_LANES + r);
+        q.a = make_int4(ap[0], ap[1], ap[2], ap[3]);
+        q.o = g_gl_i(P.out, s * HCG_GROUP_LANES + r);
     } else {
         q.op = 0;
         q.a = make_int4(0, 0, 0, 0);
@@ -43,12 +121,18 @@ __device__ __forceinline__ GSlot g_slot(const hcg_gprog* P, int s, int r) {
     return q;
 }
 
+// Points the evaluation at program P (and its constant table).
+__device__ __forceinline__ void g_bind(GEval& E, const hcg_gprog* P) {
+    E.P = P;
+    E.consts = P->consts;
+}
+
 // ------------------------------- ComplexF64 -------------------------------
 __device__ __forceinline__ cplx g_fetch_c(const GEval& E, int32_t ref) {
     const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
-    if (kind == HCG_GK_REG) return ((const cplx*)E.lds)[idx];
-    if (kind == HCG_GK_CONST) return mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]);
-    if (kind == HCG_GK_PARAM) return par_lerp(E.t, E.p, E.q, idx);
+    if (kind == HCG_GK_REG) return g_ld_c(E.lds, idx);
+    if (kind == HCG_GK_CONST) return g_gl_c(E.consts, idx);
+    if (kind == HCG_GK_PARAM) return g_par_lerp(E.t, g_gl_c(E.p, idx), g_gl_c(E.q, idx));
     return mkc(0.0, 0.0);
 }
 __device__ __forceinline__ cplx g_op_c(int op, cplx a, cplx b, cplx c, cplx d, int e) {
@@ -80,24 +164,25 @@ __device__ __forceinline__ cplx g_op_c(int op, cplx a, cplx b, cplx c, cplx d, i
 // Loads lane r's variable value into each tape's variable register, runs every slot.
 __device__ __forceinline__ void g_run_c(GEval E, int r, int n, cplx x) {
     const hcg_gprog* P = E.P;
-    cplx* R = (cplx*)E.lds;
+    char* R = E.lds;
     if (r < n)
         for (int t = 0; t < P->ntapes; t++) {
-            const int vr = P->var_reg[t][r];
-            if (vr >= 0) R[vr] = x;
+            const int vr = g_gl_i(P->var_reg[t], r);
+            if (vr >= 0) g_st_c(R, vr, x);
         }
     g_slot_end();
-    GSlot next = g_slot(P, 0, r);
-    for (int s = 0; s < P->nslots; s++) {
+    const GTables Tb = g_tables(P);
+    GSlot next = g_slot(Tb, 0, r);
+    for (int s = 0; s < Tb.nslots; s++) {
         const GSlot cur = next;
-        next = g_slot(P, s + 1, r);
+        next = g_slot(Tb, s + 1, r);
         const int op = cur.op, o = cur.o;
         const int4 a = cur.a;
         if (o >= 0) {
             cplx va = g_fetch_c(E, a.x);
             cplx vb = op == 22 ? mkc(0.0, 0.0) : g_fetch_c(E, a.y);
             cplx vc = g_fetch_c(E, a.z), vd = g_fetch_c(E, a.w);
-            R[o] = g_op_c(op, va, vb, vc, vd, g_int_imm(a.y));
+            g_st_c(R, o, g_op_c(op, va, vb, vc, vd, g_int_imm(a.y)));
         }
         g_slot_end();
     }
@@ -106,9 +191,9 @@ __device__ __forceinline__ void g_run_c(GEval E, int r, int n, cplx x) {
 // ------------------------------- ComplexDF64 -------------------------------
 __device__ __forceinline__ cdd g_fetch_dd(const GEval& E, int32_t ref) {
     const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
-    if (kind == HCG_GK_REG) return ((const cdd*)E.lds)[idx];
-    if (kind == HCG_GK_CONST) return cdd_from(mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]));
-    if (kind == HCG_GK_PARAM) return cdd_from(par_lerp(E.t, E.p, E.q, idx));
+    if (kind == HCG_GK_REG) return g_ld_dd(E.lds, idx);
+    if (kind == HCG_GK_CONST) return cdd_from(g_gl_c(E.consts, idx));
+    if (kind == HCG_GK_PARAM) return cdd_from(g_par_lerp(E.t, g_gl_c(E.p, idx), g_gl_c(E.q, idx)));
     return mkcdd(mkdd(0.0, 0.0), mkdd(0.0, 0.0));
 }
 __device__ __forceinline__ cdd g_op_dd(int op, cdd a, cdd b, cdd c, cdd d, int e) {
@@ -139,24 +224,25 @@ __device__ __forceinline__ cdd g_op_dd(int op, cdd a, cdd b, cdd c, cdd d, int e
 }
 __device__ __forceinline__ void g_run_dd(GEval E, int r, int n, cplx x) {
     const hcg_gprog* P = E.P;
-    cdd* R = (cdd*)E.lds;
+    char* R = E.lds;
     if (r < n)
         for (int t = 0; t < P->ntapes; t++) {
-            const int vr = P->var_reg[t][r];
-            if (vr >= 0) R[vr] = cdd_from(x);
+            const int vr = g_gl_i(P->var_reg[t], r);
+            if (vr >= 0) g_st_dd(R, vr, cdd_from(x));
         }
     g_slot_end();
-    GSlot next = g_slot(P, 0, r);
-    for (int s = 0; s < P->nslots; s++) {
+    const GTables Tb = g_tables(P);
+    GSlot next = g_slot(Tb, 0, r);
+    for (int s = 0; s < Tb.nslots; s++) {
         const GSlot cur = next;
-        next = g_slot(P, s + 1, r);
+        next = g_slot(Tb, s + 1, r);
         const int op = cur.op, o = cur.o;
         const int4 a = cur.a;
         if (o >= 0) {
             cdd va = g_fetch_dd(E, a.x);
             cdd vb = op == 22 ? cdd_from(mkc(0.0, 0.0)) : g_fetch_dd(E, a.y);
             cdd vc = g_fetch_dd(E, a.z), vd = g_fetch_dd(E, a.w);
-            R[o] = g_op_dd(op, va, vb, vc, vd, g_int_imm(a.y));
+            g_st_dd(R, o, g_op_dd(op, va, vb, vc, vd, g_int_imm(a.y)));
         }
         g_slot_end();
     }
@@ -165,11 +251,12 @@ __device__ __forceinline__ void g_run_dd(GEval E, int r, int n, cplx x) {
 // --------------------------- Taylor series, order K ---------------------------
 template <int K> __device__ __forceinline__ ser<K> g_fetch_s(const GEval& E, int32_t ref) {
     const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
-    if (kind == HCG_GK_REG) return ((const ser<K>*)E.lds)[idx];
-    if (kind == HCG_GK_CONST) return s_const<K>(mkc(E.P->consts[2 * idx], E.P->consts[2 * idx + 1]));
+    if (kind == HCG_GK_REG) return g_ld_s<K>(E.lds, idx);
+    if (kind == HCG_GK_CONST) return s_const<K>(g_gl_c(E.consts, idx));
     if (kind == HCG_GK_PARAM) {  // p(t + s) = p(t) + s (p - q)
-        ser<K> r = s_const<K>(par_lerp(E.t, E.p, E.q, idx));
-        r.c[1] = par_diff(E.p, E.q, idx);
+        const cplx pk = g_gl_c(E.p, idx), qk = g_gl_c(E.q, idx);
+        ser<K> r = s_const<K>(g_par_lerp(E.t, pk, qk));
+        r.c[1] = c_sub(pk, qk);  // par_diff
         return r;
     }
     return s_const<K>(mkc(0.0, 0.0));
@@ -204,28 +291,29 @@ template <int K> __device__ __forceinline__ ser<K> g_op_s(int op, const ser<K>& 
 // x0..x2: lane r's entries of the Taylor vectors of orders 0..K-1 (order K of x is zero)
 template <int K> __device__ __forceinline__ void g_run_s(GEval E, int r, int n, cplx x0, cplx x1, cplx x2) {
     const hcg_gprog* P = E.P;
-    ser<K>* R = (ser<K>*)E.lds;
+    char* R = E.lds;
     if (r < n) {
         ser<K> xs = s_const<K>(x0);
         if (K >= 2) xs.c[1] = x1;
         if (K >= 3) xs.c[2] = x2;
         for (int t = 0; t < P->ntapes; t++) {
-            const int vr = P->var_reg[t][r];
-            if (vr >= 0) R[vr] = xs;
+            const int vr = g_gl_i(P->var_reg[t], r);
+            if (vr >= 0) g_st_s<K>(R, vr, xs);
         }
     }
     g_slot_end();
-    GSlot next = g_slot(P, 0, r);
-    for (int s = 0; s < P->nslots; s++) {
+    const GTables Tb = g_tables(P);
+    GSlot next = g_slot(Tb, 0, r);
+    for (int s = 0; s < Tb.nslots; s++) {
         const GSlot cur = next;
-        next = g_slot(P, s + 1, r);
+        next = g_slot(Tb, s + 1, r);
         const int op = cur.op, o = cur.o;
         const int4 a = cur.a;
         if (o >= 0) {
             ser<K> va = g_fetch_s<K>(E, a.x);
             ser<K> vb = op == 22 ? s_const<K>(mkc(0.0, 0.0)) : g_fetch_s<K>(E, a.y);
             ser<K> vc = g_fetch_s<K>(E, a.z), vd = g_fetch_s<K>(E, a.w);
-            R[o] = g_op_s<K>(op, va, vb, vc, vd, g_int_imm(a.y));
+            g_st_s<K>(R, o, g_op_s<K>(op, va, vb, vc, vd, g_int_imm(a.y)));
         }
         g_slot_end();
     }

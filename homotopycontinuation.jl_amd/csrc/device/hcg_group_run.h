@@ -23,16 +23,16 @@ __device__ __forceinline__ void g_do_eval(GT<NN>& T) {
     const int rr = r < n ? r : 0;
     switch (T.req) {
     case GREQ_EJ: {
-        E.P = &S->ej;
+        g_bind(E, &S->ej);
         g_run_c(E, r, n, T.ev_x);
         if (S->kind == HCG_PARAMETER) {
-            T.ev_u = g_read_c(E, E.P->u_ref[0][rr]);
-            G_FOR(NN, J, T.A[J] = J < n ? g_read_c(E, E.P->U_ref[0][rr * n + (J < n ? J : 0)]) : mkc(0.0, 0.0););
+            T.ev_u = g_read_c(E, g_gl_i(E.P->u_ref[0], rr));
+            G_FOR(NN, J, T.A[J] = J < n ? g_read_c(E, g_gl_i(E.P->U_ref[0], rr * n + (J < n ? J : 0))) : mkc(0.0, 0.0););
         } else {  // u = γ t G + (1 - t) F entry by entry (src/homotopies/straight_line_homotopy.jl:95-124)
-            T.ev_u = c_muladd(ts, g_read_c(E, E.P->u_ref[0][rr]), c_mul(tt, g_read_c(E, E.P->u_ref[1][rr])));
+            T.ev_u = c_muladd(ts, g_read_c(E, g_gl_i(E.P->u_ref[0], rr)), c_mul(tt, g_read_c(E, g_gl_i(E.P->u_ref[1], rr))));
             G_FOR(NN, J, {
                 if (J < n) {
-                    const cplx gj = g_read_c(E, E.P->U_ref[0][rr * n + J]), fj = g_read_c(E, E.P->U_ref[1][rr * n + J]);
+                    const cplx gj = g_read_c(E, g_gl_i(E.P->U_ref[0], rr * n + J)), fj = g_read_c(E, g_gl_i(E.P->U_ref[1], rr * n + J));
                     T.A[J] = c_muladd(ts, gj, c_mul(tt, fj));
                 } else {
                     T.A[J] = mkc(0.0, 0.0);
@@ -43,48 +43,48 @@ __device__ __forceinline__ void g_do_eval(GT<NN>& T) {
     }
     case GREQ_E:
     case GREQ_T1:
-        E.P = &S->e;
+        g_bind(E, &S->e);
         if (T.req == GREQ_T1 && S->kind == HCG_PARAMETER) {  // taylor!(Val(1)) through the series
             g_run_s<1>(E, r, n, T.tx0, T.tx0, T.tx0);
-            T.ev_u = g_read_s<1>(E, E.P->u_ref[0][rr], 1);
+            T.ev_u = g_read_s<1>(E, g_gl_i(E.P->u_ref[0], rr), 1);
             break;
         }
         g_run_c(E, r, n, T.req == GREQ_T1 ? T.tx0 : T.ev_x);
         if (S->kind == HCG_PARAMETER) {
-            T.ev_u = g_read_c(E, E.P->u_ref[0][rr]);
+            T.ev_u = g_read_c(E, g_gl_i(E.P->u_ref[0], rr));
         } else if (T.req == GREQ_T1) {  // taylor_1!: γ G(x) - F(x)
-            T.ev_u = c_sub(c_mul(g, g_read_c(E, E.P->u_ref[0][rr])), g_read_c(E, E.P->u_ref[1][rr]));
+            T.ev_u = c_sub(c_mul(g, g_read_c(E, g_gl_i(E.P->u_ref[0], rr))), g_read_c(E, g_gl_i(E.P->u_ref[1], rr)));
         } else {
-            T.ev_u = c_muladd(ts, g_read_c(E, E.P->u_ref[0][rr]), c_mul(tt, g_read_c(E, E.P->u_ref[1][rr])));
+            T.ev_u = c_muladd(ts, g_read_c(E, g_gl_i(E.P->u_ref[0], rr)), c_mul(tt, g_read_c(E, g_gl_i(E.P->u_ref[1], rr))));
         }
         break;
     case GREQ_DD:
-        E.P = &S->e;
+        g_bind(E, &S->e);
         g_run_dd(E, r, n, T.ev_x);
         if (S->kind == HCG_PARAMETER) {
-            T.ev_u = cdd_round(g_read_dd(E, E.P->u_ref[0][rr]));
+            T.ev_u = cdd_round(g_read_dd(E, g_gl_i(E.P->u_ref[0], rr)));
         } else {
-            T.ev_u = cdd_round(cdd_add(cdd_mul_c(ts, g_read_dd(E, E.P->u_ref[0][rr])),
-                                       cdd_mul_c(tt, g_read_dd(E, E.P->u_ref[1][rr]))));
+            T.ev_u = cdd_round(cdd_add(cdd_mul_c(ts, g_read_dd(E, g_gl_i(E.P->u_ref[0], rr))),
+                                       cdd_mul_c(tt, g_read_dd(E, g_gl_i(E.P->u_ref[1], rr)))));
         }
         break;
     case GREQ_T2:
     case GREQ_T3: {
-        E.P = &S->e;
+        g_bind(E, &S->e);
         const int K = T.req == GREQ_T2 ? 2 : 3;
         cplx gK1, gK, fK1, fK;
         if (K == 2) {
             g_run_s<2>(E, r, n, T.tx0, T.tx1, T.tx1);
-            fK = g_read_s<2>(E, E.P->u_ref[E.P->ntapes - 1][rr], 2);
-            fK1 = g_read_s<2>(E, E.P->u_ref[E.P->ntapes - 1][rr], 1);
-            gK = S->kind == HCG_PARAMETER ? fK : g_read_s<2>(E, E.P->u_ref[0][rr], 2);
-            gK1 = S->kind == HCG_PARAMETER ? fK1 : g_read_s<2>(E, E.P->u_ref[0][rr], 1);
+            fK = g_read_s<2>(E, g_gl_i(E.P->u_ref[E.P->ntapes - 1], rr), 2);
+            fK1 = g_read_s<2>(E, g_gl_i(E.P->u_ref[E.P->ntapes - 1], rr), 1);
+            gK = S->kind == HCG_PARAMETER ? fK : g_read_s<2>(E, g_gl_i(E.P->u_ref[0], rr), 2);
+            gK1 = S->kind == HCG_PARAMETER ? fK1 : g_read_s<2>(E, g_gl_i(E.P->u_ref[0], rr), 1);
         } else {
             g_run_s<3>(E, r, n, T.tx0, T.tx1, T.tx2);
-            fK = g_read_s<3>(E, E.P->u_ref[E.P->ntapes - 1][rr], 3);
-            fK1 = g_read_s<3>(E, E.P->u_ref[E.P->ntapes - 1][rr], 2);
-            gK = S->kind == HCG_PARAMETER ? fK : g_read_s<3>(E, E.P->u_ref[0][rr], 3);
-            gK1 = S->kind == HCG_PARAMETER ? fK1 : g_read_s<3>(E, E.P->u_ref[0][rr], 2);
+            fK = g_read_s<3>(E, g_gl_i(E.P->u_ref[E.P->ntapes - 1], rr), 3);
+            fK1 = g_read_s<3>(E, g_gl_i(E.P->u_ref[E.P->ntapes - 1], rr), 2);
+            gK = S->kind == HCG_PARAMETER ? fK : g_read_s<3>(E, g_gl_i(E.P->u_ref[0], rr), 3);
+            gK1 = S->kind == HCG_PARAMETER ? fK1 : g_read_s<3>(E, g_gl_i(E.P->u_ref[0], rr), 2);
         }
         if (S->kind == HCG_PARAMETER) {
             T.ev_u = fK;
