@@ -874,8 +874,12 @@ struct GT {
     // the network of hcg_mlp_infer.h evaluated by the 16 lanes together (each lane 4 hidden
     // units), with every sum in the order of mlp_forward, so the output is identical
     __device__ __forceinline__ void mlp_forward_group(float* out) {
-        float* hbuf = (float*)(lds) ;  // 128 floats of this group's register file (free between evaluations)
-        const unsigned short* wv = mlp_w;
+        // 128 floats of this group's register file (free between evaluations); both the
+        // hidden units and the staged weights are read through local (LDS) pointers
+        typedef __attribute__((address_space(3))) float lds_f32;
+        typedef __attribute__((address_space(3))) const unsigned short lds_u16;
+        lds_f32* hbuf = (lds_f32*)lds;
+        lds_u16* wv = (lds_u16*)mlp_w;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int j = 4 * r + u;
