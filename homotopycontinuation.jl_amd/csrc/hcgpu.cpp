@@ -920,7 +920,9 @@ extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
         HIPCHK(hcg::group_kernel_attributes(p->group->nn, &fa));
         out6[0] = fa.numRegs;
         out6[1] = (int32_t)fa.localSizeBytes;
-        out6[2] = hcg::group_mlp_lds_bytes() + (hcg::group_block() / HCG_GROUP_LANES) * p->group->lds_per_path;
+        // LDS of a plain launch; the learned step stages its network's weights in front of it
+        // (group_mlp_lds_bytes() more per workgroup)
+        out6[2] = (hcg::group_block() / HCG_GROUP_LANES) * p->group->lds_per_path;
         int blocks = 0;  // workgroups per CU; each holds one wave per SIMD
         HIPCHK(hcg::group_kernel_blocks_per_cu(p->group->nn, (size_t)out6[2], &blocks));
         out6[3] = blocks;

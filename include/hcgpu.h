@@ -205,7 +205,8 @@ int hcg_last_counters(hcg_ctx* ctx, uint64_t* out4);
 /* Resource use of a program's kernel from its code object (no counterpart in the reference:
  * the occupancy evidence of the measurement): out6 = [VGPRs per lane, private segment bytes
  * per lane, static LDS bytes per workgroup, waves per SIMD the kernel was built for (lane-group
- * layout: workgroups that fit on a CU, each one wave per SIMD),
+ * layout: LDS of a plain launch, to which the learned step adds its network's BF16 weights,
+ * and the workgroups that fit on a CU with it, each one wave per SIMD),
  * workgroup size, compute units of the device]. */
 int hcg_program_kernel_info(const hcg_program* prog, int32_t* out6);
 /* The lane-group layout's slot schedules of nsys homotopies, built and checked against their
