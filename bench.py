@@ -12,7 +12,7 @@ restatement of the reference tracker) on a bounded sample of the same batch.
 Other workloads (--workload): "dense4" = 4 dense cubics in 4 unknowns, 81 total-degree paths;
 "dense8" = 8 dense cubics in 8 unknowns, 6,561 total-degree paths; "mixed" = shared-factor
 systems of n = 4, 6, 8, 12 (538,812 paths) in one launch; "steiner" = the Steiner conics
-(n = 15) tracked to 8,192 nearby targets.  The default line also carries the
+(n = 15) tracked to 8,192 nearby targets; "fourbar" = the four-bar linkage (n = 24) likewise.  The default line also carries the
 mixed workload's learned and plain launches (`mixed_workload`).
 
 The JSON line carries the counted-FLOP roofline of the timed kernel (HIP events on the
@@ -462,17 +462,31 @@ def bench_dense(args, out):
         out["parity_vs_oracle"] = {"identical_return_codes": bool((o.code == rec["code"]).all())}
 
 
+def nearby_targets(q, m, seed):
+    """m targets near q: every coefficient perturbed by 1e-3 relative (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.asarray(q)[None, :] * (1 + 1e-3 * (rng.standard_normal((m, len(q))) + 1j * rng.standard_normal((m, len(q)))))
+
+
 def steiner_problem(m, seed=5):
     """Steiner conics (n = 15, 30 parameters; test/test_cases/steiner_higher_prec.jl): the
-    reference's start solution s_p at p tracked to m targets near its q (every coefficient
-    perturbed by 1e-3 relative, seeded)."""
+    reference's start solution s_p at p tracked to m targets near its q."""
     import numpy as np
     sys.path.insert(0, str(REPO / "tests"))
     import systems
     F, p, q, s_p, _ = systems.steiner()
-    rng = np.random.default_rng(seed)
-    Q = np.asarray(q)[None, :] * (1 + 1e-3 * (rng.standard_normal((m, len(q))) + 1j * rng.standard_normal((m, len(q)))))
-    return F, np.asarray(p), Q, np.asarray(s_p)
+    return F, np.asarray(p), nearby_targets(q, m, seed), np.asarray(s_p)
+
+
+def fourbar_problem(m, seed=6):
+    """Four-bar linkage synthesis (n = 24, 16 parameters; test/test_cases/four_bar.jl): the
+    reference's start point s (8 digits) at p tracked to m targets near its q."""
+    import numpy as np
+    sys.path.insert(0, str(REPO / "tests"))
+    import systems
+    F, p, q, s = systems.four_bar()
+    return F, np.asarray(p), nearby_targets(q, m, seed), np.asarray(s)
 
 
 def bench_steiner(args, out):
@@ -482,13 +496,18 @@ def bench_steiner(args, out):
     from hcgpu.homotopies import ParameterHomotopy
     from hcgpu.solve import make_options
     m = 8192
-    F, p, Q, s_p = steiner_problem(m)
+    fourbar = args.workload == "fourbar"
+    F, p, Q, s_p = fourbar_problem(m) if fourbar else steiner_problem(m)
     H = ParameterHomotopy(F, p, Q)
     S = np.tile(s_p[None, :], (m, 1))
     groups = np.arange(m, dtype=np.int32)
     ctx = GpuContext(0)
     opts = make_options()
-    if args.layout == "auto":
+    if fourbar:
+        # n = 24: the lane-group kernel holds at most 16 variables; the per-lane kernel as built
+        prog = ctx.program(H, layout="lane")
+        launch = {"chosen_by": "only layout for n = 24"}
+    elif args.layout == "auto":
         # n = 15 is past the per-lane kernel's register-resident LU (n <= 12): the tuner times the
         # lane-group kernel beside it
         cands = autotune.candidates_for(H.n, lane=((1, 4), (1, 8), (2, 8)), group=(1, 2))
@@ -506,10 +525,12 @@ def bench_steiner(args, out):
     r = timed(batch, opts, args.steps, args.warmup)
     rec, sol = batch.results()[:2]
     out.update({"value": round(m * args.steps / r["wall"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 3),
-                "config": {"workload": f"steiner: n=15, 30 parameters, the reference's Steiner start tracked to {m} "
-                                       "targets near its q in one persistent launch, plain tracker", "n": 15,
+                "config": {"workload": (f"fourbar: n=24, 16 parameters, the reference's four-bar start tracked to {m} "
+                                        "targets near its q in one persistent launch, plain tracker") if fourbar else
+                                       (f"steiner: n=15, 30 parameters, the reference's Steiner start tracked to {m} "
+                                        "targets near its q in one persistent launch, plain tracker"), "n": H.n,
                            "paths": m, "launch": launch},
-                "roofline": roofline(r["flops"], r["kernel_ms"], f"steiner_{m}paths" + ("" if prog.layout == "lane" else "_group"),
+                "roofline": roofline(r["flops"], r["kernel_ms"], ("fourbar" if fourbar else "steiner") + f"_{m}paths" + ("" if prog.layout == "lane" else "_group"),
                                      kernel="hcg_track_kernel" if prog.layout == "lane" else "hcg_group_kernel",
                                      info=prog.kernel_info(), grid_waves=grid_waves(prog, m)),
                 "steps_per_path": round(r["steps"] / m, 3), "solution_counts": counts(rec)})
@@ -628,7 +649,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["param10", "dense4", "dense8", "mixed", "steiner"], default="param10")
+    ap.add_argument("--workload", choices=["param10", "dense4", "dense8", "mixed", "steiner", "fourbar"], default="param10")
     ap.add_argument("--retune", action="store_true", help="tune again even when the autotuner has a cached choice")
     ap.add_argument("--layout", choices=["auto", "lane", "group"], default="auto",
                     help="auto: the autotuner's choice (param10, dense4, dense8, steiner; the mixed workload takes lane); lane: one path "
@@ -666,7 +687,7 @@ def main():
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded complex Gaussian coefficients)"}
     {"param10": bench_param10, "dense4": bench_dense, "dense8": bench_dense, "mixed": bench_mixed,
-     "steiner": bench_steiner}[args.workload](args, out)
+     "steiner": bench_steiner, "fourbar": bench_steiner}[args.workload](args, out)
     if args.workload == "param10" and not args.no_mixed:
         out["mixed_workload"] = mixed_leg(args)
     print(json.dumps(out))
