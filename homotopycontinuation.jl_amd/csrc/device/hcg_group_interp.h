@@ -135,6 +135,31 @@ __device__ __forceinline__ cplx g_fetch_c(const GEval& E, int32_t ref) {
     if (kind == HCG_GK_PARAM) return g_par_lerp(E.t, g_gl_c(E.p, idx), g_gl_c(E.q, idx));
     return mkc(0.0, 0.0);
 }
+// The constants and parameter values p_k, q_k a slot entry's operands need, loaded one slot
+// ahead (no load for register and immediate operands, nor for lanes without an instruction).
+struct GVals {
+    cplx p[4], q[4];
+};
+__device__ __forceinline__ void g_vals(const GEval& E, const GSlot& sl, GVals& v) {
+    const int32_t ref[4] = {sl.a.x, sl.a.y, sl.a.z, sl.a.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int kind = HCG_GREF_KIND(ref[k]), idx = HCG_GREF_INDEX(ref[k]);
+        if (sl.o >= 0 && kind == HCG_GK_CONST) v.p[k] = g_gl_c(E.consts, idx);
+        if (sl.o >= 0 && kind == HCG_GK_PARAM) {
+            v.p[k] = g_gl_c(E.p, idx);
+            v.q[k] = g_gl_c(E.q, idx);
+        }
+    }
+}
+// g_fetch_c with the constant or parameter values already loaded
+__device__ __forceinline__ cplx g_fetch_cv(const GEval& E, int32_t ref, cplx pk, cplx qk) {
+    const int kind = HCG_GREF_KIND(ref), idx = HCG_GREF_INDEX(ref);
+    if (kind == HCG_GK_REG) return g_ld_c(E.lds, idx);
+    if (kind == HCG_GK_CONST) return pk;
+    if (kind == HCG_GK_PARAM) return g_par_lerp(E.t, pk, qk);
+    return mkc(0.0, 0.0);
+}
 __device__ __forceinline__ cplx g_op_c(int op, cplx a, cplx b, cplx c, cplx d, int e) {
     switch (op) {
     case 1: return c_cb(a);
@@ -171,17 +196,25 @@ __device__ __forceinline__ void g_run_c(GEval E, int r, int n, cplx x) {
             if (vr >= 0) g_st_c(R, vr, x);
         }
     g_slot_end();
+    // slot s runs with its table entries loaded two slots ahead and its constants and
+    // parameter values one slot ahead, so no global load latency lies on the slot chain
     const GTables Tb = g_tables(P);
     GSlot next = g_slot(Tb, 0, r);
+    GVals nv;
+    g_vals(E, next, nv);
+    GSlot next2 = g_slot(Tb, 1, r);
     for (int s = 0; s < Tb.nslots; s++) {
         const GSlot cur = next;
-        next = g_slot(Tb, s + 1, r);
+        const GVals cv = nv;
+        next = next2;
+        g_vals(E, next, nv);
+        next2 = g_slot(Tb, s + 2, r);
         const int op = cur.op, o = cur.o;
         const int4 a = cur.a;
         if (o >= 0) {
-            cplx va = g_fetch_c(E, a.x);
-            cplx vb = op == 22 ? mkc(0.0, 0.0) : g_fetch_c(E, a.y);
-            cplx vc = g_fetch_c(E, a.z), vd = g_fetch_c(E, a.w);
+            cplx va = g_fetch_cv(E, a.x, cv.p[0], cv.q[0]);
+            cplx vb = op == 22 ? mkc(0.0, 0.0) : g_fetch_cv(E, a.y, cv.p[1], cv.q[1]);
+            cplx vc = g_fetch_cv(E, a.z, cv.p[2], cv.q[2]), vd = g_fetch_cv(E, a.w, cv.p[3], cv.q[3]);
             g_st_c(R, o, g_op_c(op, va, vb, vc, vd, g_int_imm(a.y)));
         }
         g_slot_end();
