@@ -97,10 +97,11 @@ def roofline(flops, kernel_ms, workload, kernel="hcg_track_kernel", info=None, g
 def grid_waves(prog, npaths):
     """Waves the launch starts.  Per-lane kernel: one per 64 paths, at most one per wave slot
     (4 SIMDs x the waves per SIMD it was built for, per CU).  Lane-group kernel: four per
-    workgroup of 16 paths, at most two workgroups per CU."""
+    workgroup of 16 paths (8 paths with 32-lane groups, n > 16), at most two workgroups per CU."""
     info = prog.kernel_info()
     if getattr(prog, "layout", "lane") == "group":
-        return 4 * min(-(-npaths // 16), 2 * info["cus"])
+        per_wg = 16 if prog.n <= 16 else 8
+        return 4 * min(-(-npaths // per_wg), 2 * info["cus"])
     return min(-(-npaths // 64), 4 * info["cus"] * info["waves_per_simd_built"])
 
 
@@ -504,9 +505,10 @@ def bench_steiner(args, out):
     ctx = GpuContext(0)
     opts = make_options()
     if fourbar:
-        # n = 24: the lane-group kernel holds at most 16 variables; the per-lane kernel as built
-        prog = ctx.program(H, layout="lane")
-        launch = {"chosen_by": "only layout for n = 24"}
+        # n = 24: the 32-lane group kernel unless --layout lane.  The tuner is not run: it times
+        # every candidate twice on the whole batch, and the per-lane kernel takes ~39 s here
+        prog = ctx.program(H, layout="lane" if args.layout == "lane" else "group")
+        launch = {"chosen_by": "--layout" if args.layout != "auto" else "default for n = 24 (lane-group, 32 lanes)"}
     elif args.layout == "auto":
         # n = 15 is past the per-lane kernel's register-resident LU (n <= 12): the tuner times the
         # lane-group kernel beside it

@@ -880,8 +880,11 @@ struct GT {
         typedef __attribute__((address_space(3))) const unsigned short lds_u16;
         lds_f32* hbuf = (lds_f32*)lds;
         lds_u16* wv = (lds_u16*)mlp_w;
+        // lanes 0..15 hold 4 hidden units each (in a 32-lane group lanes 16..31 have none)
+        const bool has_units = r < HCG_MLP_HIDDEN / 4;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
+            if (!has_units) break;
             const int j = 4 * r + u;
             float s = bf16_to_f32(wv[MLP_B1 + j]);
 #pragma unroll
@@ -893,6 +896,7 @@ struct GT {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int u = 0; u < 4; u++) {
+            if (!has_units) break;
             const int j = 4 * r + u;
             float s = bf16_to_f32(wv[MLP_B2 + j]);
             for (int k = 0; k < HCG_MLP_HIDDEN; k++) s = fmaf(bf16_to_f32(wv[MLP_W2 + j * HCG_MLP_HIDDEN + k]), hbuf[k], s);

@@ -108,7 +108,7 @@ __device__ __forceinline__ GTables g_tables(const hcg_gprog* P) {
 }
 __device__ __forceinline__ GSlot g_slot(const GTables& P, int s, int r) {
     GSlot q;
-    if (s < P.nslots) {
+    if (s < P.nslots && r < HCG_GROUP_LANES) {  // a slot holds 16 instructions: lanes 16..31 of a 32-lane group idle
         q.op = ((const g_glob_u8*)P.op)[s];
         const g_glob_i* ap = (const g_glob_i*)P.arg + 4 * (s * HCG_GROUP_LANES + r);
         q.a = make_int4(ap[0], ap[1], ap[2], ap[3]);

@@ -1,6 +1,7 @@
-// Cross-lane primitives of the lane-group tracker: one path per group of 16 consecutive lanes
-// of a wave (a DPP row), lane r of the group holding entry r of every vector and row r of
-// the Jacobian.
+// Cross-lane primitives of the lane-group tracker: one path per group of GL consecutive lanes
+// of a wave, lane r of the group holding entry r of every vector and row r of the Jacobian.
+// GL = 16 (a DPP row; systems of up to 16 variables) unless the including file defines
+// GL 32 (half a wave; up to 24 variables, the library's largest system size).
 //
 // Reductions that the reference writes as sequential loops over the entries (sums, maxima
 // with its NaN conventions, first-index searches) are done as the same sequential loop over
@@ -8,14 +9,23 @@
 #pragma once
 #include "hcg_arith.h"
 
+#ifndef GL
 #define GL 16
+#endif
+static_assert(GL == 16 || GL == 32, "lane groups of 16 or 32 lanes");
+#define GL_MASK ((GL == 32) ? 0xFFFFFFFFull : 0xFFFFull)
 
-__device__ __forceinline__ int g_lane() { return (int)(__lane_id() & 15u); }
-__device__ __forceinline__ int g_base() { return (int)(__lane_id() & ~15u); }
+__device__ __forceinline__ int g_lane() { return (int)(__lane_id() & (unsigned)(GL - 1)); }
+__device__ __forceinline__ int g_base() { return (int)(__lane_id() & ~(unsigned)(GL - 1)); }
 
-// broadcast of lane J (compile time) of the group: DPP row_newbcast
+// broadcast of lane J (compile time) of the group: DPP row_newbcast within a 16-lane row;
+// across the two rows of a 32-lane group, a permute through the LDS crossbar
 template <int J> __device__ __forceinline__ int gb_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+    if constexpr (GL == 16) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x150 + J, 0xF, 0xF, false);
+    } else {
+        return __shfl(v, g_base() + J, 64);
+    }
 }
 template <int J> __device__ __forceinline__ double gb_d(double v) {
     int2 u = __builtin_bit_cast(int2, v);
@@ -33,7 +43,7 @@ __device__ __forceinline__ cplx gs_c(cplx v, int j) { return mkc(gs_d(v.re, j), 
 // any / all over the group's first n lanes
 __device__ __forceinline__ bool g_any(bool p, int n) {
     const unsigned long long m = __ballot(p && g_lane() < n ? 1 : 0);
-    return ((m >> g_base()) & 0xFFFFull) != 0ull;
+    return ((m >> g_base()) & GL_MASK) != 0ull;
 }
 __device__ __forceinline__ bool g_all(bool p, int n) { return !g_any(!p, n); }
 
@@ -45,6 +55,10 @@ template <int NN, class Fn> __device__ __forceinline__ double g_fold(double v, i
     HCG_GFOLD_STEP(1) HCG_GFOLD_STEP(2) HCG_GFOLD_STEP(3) HCG_GFOLD_STEP(4) HCG_GFOLD_STEP(5)
     HCG_GFOLD_STEP(6) HCG_GFOLD_STEP(7) HCG_GFOLD_STEP(8) HCG_GFOLD_STEP(9) HCG_GFOLD_STEP(10)
     HCG_GFOLD_STEP(11) HCG_GFOLD_STEP(12) HCG_GFOLD_STEP(13) HCG_GFOLD_STEP(14) HCG_GFOLD_STEP(15)
+    HCG_GFOLD_STEP(16) HCG_GFOLD_STEP(17) HCG_GFOLD_STEP(18) HCG_GFOLD_STEP(19) HCG_GFOLD_STEP(20)
+    HCG_GFOLD_STEP(21) HCG_GFOLD_STEP(22) HCG_GFOLD_STEP(23) HCG_GFOLD_STEP(24) HCG_GFOLD_STEP(25)
+    HCG_GFOLD_STEP(26) HCG_GFOLD_STEP(27) HCG_GFOLD_STEP(28) HCG_GFOLD_STEP(29) HCG_GFOLD_STEP(30)
+    HCG_GFOLD_STEP(31)
 #undef HCG_GFOLD_STEP
     return acc;
 }
@@ -55,12 +69,16 @@ template <int NN> __device__ __forceinline__ double g_sum(double v, int n, doubl
     HCG_GSUM_STEP(0) HCG_GSUM_STEP(1) HCG_GSUM_STEP(2) HCG_GSUM_STEP(3) HCG_GSUM_STEP(4) HCG_GSUM_STEP(5)
     HCG_GSUM_STEP(6) HCG_GSUM_STEP(7) HCG_GSUM_STEP(8) HCG_GSUM_STEP(9) HCG_GSUM_STEP(10)
     HCG_GSUM_STEP(11) HCG_GSUM_STEP(12) HCG_GSUM_STEP(13) HCG_GSUM_STEP(14) HCG_GSUM_STEP(15)
+    HCG_GSUM_STEP(16) HCG_GSUM_STEP(17) HCG_GSUM_STEP(18) HCG_GSUM_STEP(19) HCG_GSUM_STEP(20)
+    HCG_GSUM_STEP(21) HCG_GSUM_STEP(22) HCG_GSUM_STEP(23) HCG_GSUM_STEP(24) HCG_GSUM_STEP(25)
+    HCG_GSUM_STEP(26) HCG_GSUM_STEP(27) HCG_GSUM_STEP(28) HCG_GSUM_STEP(29) HCG_GSUM_STEP(30)
+    HCG_GSUM_STEP(31)
 #undef HCG_GSUM_STEP
     return acc;
 }
 // index of the first lane (0..n-1) whose predicate holds, or n
 __device__ __forceinline__ int g_first(bool p, int n) {
-    const unsigned long long m = (__ballot(p && g_lane() < n ? 1 : 0) >> g_base()) & 0xFFFFull;
+    const unsigned long long m = (__ballot(p && g_lane() < n ? 1 : 0) >> g_base()) & GL_MASK;
     return m ? __builtin_ctzll(m) : n;
 }
 

@@ -188,7 +188,7 @@ __device__ __forceinline__ void g_write_invalid(int r, int n, const hcg_kargs& a
     }
 }
 
-// The work of one lane of a workgroup: lanes 16g..16g+15 (g = tid / 16) form a group that
+// The work of one lane of a workgroup: lanes GL g .. GL g + GL - 1 (g = tid / GL) form a group that
 // tracks paths from the launch's queue until it is empty; smem: the workgroup's LDS (the
 // network's BF16 weights, then lds_per_path bytes per group).
 template <int NN>
@@ -199,7 +199,7 @@ __device__ __forceinline__ void hcg_group_run(const hcg_multi_kargs& m, const hc
     GT<NN> T;
     T.r = g_lane();
     // the step network's weights take the front of LDS only when the learned step is on
-    T.lds = smem + (mw ? HCG_GROUP_MLP_LDS : 0) + (tid >> 4) * lds_per_path;
+    T.lds = smem + (mw ? HCG_GROUP_MLP_LDS : 0) + (tid / GL) * lds_per_path;
     T.mlp_w = mw;
     T.o = opts;
     T.cond_H_xd = 1.0;

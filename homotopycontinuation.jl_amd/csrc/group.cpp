@@ -72,8 +72,10 @@ int group_nn_for(int n) {
     if (n <= 8) return 8;
     if (n <= 12) return 12;
     if (n <= 16) return 16;
+    if (n <= 24) return 24;  // the library's largest system size
     return 0;
 }
+int group_lanes_for(int nn) { return nn > 16 ? 32 : HCG_GROUP_LANES; }
 int group_mlp_lds_bytes() { return ((HCG_MLP_NPARAMS * 2 + 15) / 16) * 16; }  // BF16 weights, 16-byte aligned
 int group_block() { return 256; }
 
@@ -88,7 +90,7 @@ bool group_build(const hcg_homotopy_desc* const* hs, int nsys, bool upload, Grou
         if (!h) { *err = "homotopy is NULL"; return false; }
         if (!homotopy_flops(h, &fl[6 * s], err)) return false;
         const int n = h->n;
-        if (n > 16) { *err = "the lane-group layout tracks systems of up to 16 variables"; return false; }
+        if (n > 24) { *err = "the lane-group layout tracks systems of up to 24 variables"; return false; }
         maxn = std::max(maxn, n);
         std::vector<const hcg_tape*> tj, te;
         std::vector<const double*> fx;
@@ -115,7 +117,7 @@ bool group_build(const hcg_homotopy_desc* const* hs, int nsys, bool upload, Grou
     out->nn = group_nn_for(maxn);
     out->lds_per_path = (lds + 15) & ~15;
     out->nsys = nsys;
-    const size_t lds_total = (size_t)group_mlp_lds_bytes() + (size_t)(group_block() / HCG_GROUP_LANES) * out->lds_per_path;
+    const size_t lds_total = (size_t)group_mlp_lds_bytes() + (size_t)(group_block() / group_lanes_for(out->nn)) * out->lds_per_path;
     if (lds_total > 160 * 1024) {
         *err = "the slot programs need " + std::to_string(out->lds_per_path) + " bytes of LDS per path, more than a "
                "workgroup of 16 paths can hold";

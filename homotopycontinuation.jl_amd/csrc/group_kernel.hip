@@ -33,6 +33,7 @@ template <int NN> hipError_t allow_lds(size_t bytes) {
 
 hipError_t group_launch(int nn, int grid, int lds_per_path, bool mlp_lds, hipStream_t stream,
                         const hcg_multi_kargs& m, const hcg_gsys* d_sys) {
+    if (nn > 16) return group_launch32(nn, grid, lds_per_path, mlp_lds, stream, m, d_sys);
     // LDS: the step network's BF16 weights (learned step only), then each group's registers
     const size_t shmem = (mlp_lds ? (size_t)kMlpLdsBytes : 0) + (size_t)(kBlock / HCG_GROUP_LANES) * lds_per_path;
     hipError_t e = nn == 4 ? allow_lds<4>(shmem) : nn == 8 ? allow_lds<8>(shmem) : nn == 12 ? allow_lds<12>(shmem)
@@ -51,6 +52,7 @@ hipError_t group_launch(int nn, int grid, int lds_per_path, bool mlp_lds, hipStr
 // Workgroups of the kernel instance that fit on one CU with `shmem` bytes of dynamic LDS each
 // (registers and LDS both counted); each workgroup is one wave per SIMD.
 hipError_t group_kernel_blocks_per_cu(int nn, size_t shmem, int* blocks) {
+    if (nn > 16) return group_kernel32_blocks_per_cu(nn, shmem, blocks);
     hipError_t e = nn == 4 ? allow_lds<4>(shmem) : nn == 8 ? allow_lds<8>(shmem) : nn == 12 ? allow_lds<12>(shmem)
                                                                               : allow_lds<16>(shmem);
     if (e != hipSuccess) return e;
@@ -64,6 +66,7 @@ hipError_t group_kernel_blocks_per_cu(int nn, size_t shmem, int* blocks) {
 }
 
 hipError_t group_kernel_attributes(int nn, hipFuncAttributes* attr) {
+    if (nn > 16) return group_kernel32_attributes(nn, attr);
     switch (nn) {
     case 4: return hipFuncGetAttributes(attr, reinterpret_cast<const void*>(&hcg_group_kernel<4>));
     case 8: return hipFuncGetAttributes(attr, reinterpret_cast<const void*>(&hcg_group_kernel<8>));

@@ -13,6 +13,8 @@ namespace hcg {
 
 // Compile-time system size of the kernel instance that tracks systems of size n (0: none).
 int group_nn_for(int n);
+// Lanes per group of a kernel instance: 16 up to nn = 16, 32 for nn = 24.
+int group_lanes_for(int nn);
 int group_mlp_lds_bytes();
 int group_block();
 

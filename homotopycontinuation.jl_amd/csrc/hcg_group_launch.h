@@ -1,4 +1,4 @@
-// Launcher of the precompiled lane-group tracker kernel (group_kernel.hip).
+// Launcher of the precompiled lane-group tracker kernels (group_kernel.hip, group_kernel32.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,5 +12,10 @@ hipError_t group_launch(int nn, int grid, int lds_per_path, bool mlp_lds, hipStr
                         const hcg_multi_kargs& m, const hcg_gsys* d_sys);
 hipError_t group_kernel_attributes(int nn, hipFuncAttributes* attr);
 hipError_t group_kernel_blocks_per_cu(int nn, size_t shmem, int* blocks);
+// the 32-lane instance (nn = 24; group_kernel32.hip), called by the three above
+hipError_t group_launch32(int nn, int grid, int lds_per_path, bool mlp_lds, hipStream_t stream,
+                          const hcg_multi_kargs& m, const hcg_gsys* d_sys);
+hipError_t group_kernel32_attributes(int nn, hipFuncAttributes* attr);
+hipError_t group_kernel32_blocks_per_cu(int nn, size_t shmem, int* blocks);
 
 }  // namespace hcg

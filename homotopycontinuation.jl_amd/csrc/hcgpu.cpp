@@ -468,7 +468,7 @@ static int ctx_scalars(hcg_ctx* ctx) {
 // blocks_per_cu workgroups per CU (default 2).
 static int group_launch(hcg_ctx* ctx, const hcg_program* prog, const hcg_multi_kargs& m, long total,
                         const hcg_options* opts, hipStream_t stream) {
-    const int per_block = hcg::group_block() / HCG_GROUP_LANES;
+    const int per_block = hcg::group_block() / hcg::group_lanes_for(prog->group->nn);
     const int per_cu = opts->blocks_per_cu > 0 ? opts->blocks_per_cu : 2;
     long grid = std::min<long>((total + per_block - 1) / per_block, (long)ctx->ncu * per_cu);
     if (grid < 1) grid = 1;
@@ -922,7 +922,7 @@ extern "C" int hcg_program_kernel_info(const hcg_program* p, int32_t* out6) {
         out6[1] = (int32_t)fa.localSizeBytes;
         // LDS of a plain launch; the learned step stages its network's weights in front of it
         // (group_mlp_lds_bytes() more per workgroup)
-        out6[2] = (hcg::group_block() / HCG_GROUP_LANES) * p->group->lds_per_path;
+        out6[2] = (hcg::group_block() / hcg::group_lanes_for(p->group->nn)) * p->group->lds_per_path;
         int blocks = 0;  // workgroups per CU; each holds one wave per SIMD
         HIPCHK(hcg::group_kernel_blocks_per_cu(p->group->nn, (size_t)out6[2], &blocks));
         out6[3] = blocks;

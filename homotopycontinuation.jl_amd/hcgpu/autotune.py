@@ -2,8 +2,9 @@
 
 For a homotopy and a representative sample of its batch, times every candidate and keeps the
 fastest.  A candidate is a layout (one path per lane, or one path per 16-lane group for
-systems of up to 16 variables), the waves per SIMD the per-lane kernel is compiled for (its
-register budget), and the workgroups per CU of the persistent launch.  Choices are cached in a
+systems of up to 16 variables, per 32-lane group up to 24), the waves per SIMD the per-lane
+kernel is compiled for (its register budget), and the workgroups per CU of the persistent
+launch.  Choices are cached in a
 JSON file next to libhcgpu.so, keyed by the program's generated source and the device, so a
 workload is tuned once; GpuContext.program() and track_batch() then use the cached choice by
 default.  Tuning never changes results: every candidate runs the same arithmetic (both layouts
@@ -25,7 +26,7 @@ from .gpu import GpuContext, track_batch
 
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = ((1, 4), (1, 8), (2, 8), (2, 16), (4, 16), (4, 32), (8, 32))
 GROUP_CANDIDATES: Tuple[int, ...] = (1, 2)  # workgroups per CU of the lane-group kernel
-GROUP_MAX_N = 16
+GROUP_MAX_N = 24
 
 
 def candidates_for(n: int, lane=DEFAULT_CANDIDATES, group=GROUP_CANDIDATES) -> List[dict]:

@@ -28,7 +28,7 @@ class GpuContext:
     def program(self, homotopy, waves_per_eu: int = 0, layout: Optional[str] = None) -> "Program":
         """layout "lane": one path per lane, the tapes compiled with hipRTC; "group": one path
         per group of 16 lanes on the precompiled lane-group kernel (systems of up to 16
-        variables).  With neither layout nor waves_per_eu given, the autotuner's cached choice
+        variables; 32 lanes for 17 to 24 variables).  With neither layout nor waves_per_eu given, the autotuner's cached choice
         for this homotopy on this device (hcgpu.autotune.tune) when there is one, including
         its workgroups per CU, which track_batch then uses; otherwise one path per lane at the
         library's default register budget."""

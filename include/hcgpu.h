@@ -173,9 +173,9 @@ int hcg_compile_multi_only(const hcg_homotopy_desc* const* hs, int32_t nsys, siz
  * waves_per_eu: waves per SIMD the tracker kernel is compiled for, i.e. a register budget
  * of 512 / waves_per_eu per lane (1, 2, 4 or 8; 0 = the library default; per-lane layout).
  * layout: HCG_LAYOUT_LANE (one path per lane, the tapes compiled with hipRTC) or
- * HCG_LAYOUT_GROUP (one path per group of 16 lanes, each lane holding one row of the
- * Jacobian and one entry of every vector in registers; the tapes are scheduled for a
- * precompiled kernel, so no compilation happens; systems of up to 16 variables). */
+ * HCG_LAYOUT_GROUP (one path per group of 16 lanes, 32 lanes for 17 to 24 variables, each
+ * lane holding one row of the Jacobian and one entry of every vector in registers; the tapes
+ * are scheduled for a precompiled kernel, so no compilation happens). */
 #define HCG_LAYOUT_LANE 0
 #define HCG_LAYOUT_GROUP 1
 typedef struct hcg_build_options {

@@ -468,7 +468,7 @@ end
 
 Compile `T`'s homotopy for the GPU (`hcg_program_create_ex`): `layout = :lane` generates and
 compiles the tapes (one path per lane), `:group` schedules them for the precompiled lane-group
-kernel (one path per 16 lanes, systems of up to 16 variables).  `learned_weights`
+kernel (one path per 16 lanes, 32 lanes for 17 to 24 variables).  `learned_weights`
 (`HCG_MLP_NPARAMS` Float32) turns on the learned step proposer.
 """
 function GPUTracker(T::HC.EndgameTracker; device::Integer = 0, layout::Symbol = :lane,

@@ -2,13 +2,13 @@
 from hcgpu import autotune
 
 
-def test_layout_is_a_candidate_up_to_16_variables():
+def test_layout_is_a_candidate_up_to_24_variables():
     small = autotune.candidates_for(10)
     assert {c["layout"] for c in small} == {"lane", "group"}
     assert all(c["waves_per_eu"] == 0 for c in small if c["layout"] == "group")
     assert len([c for c in small if c["layout"] == "lane"]) == len(autotune.DEFAULT_CANDIDATES)
-    big = autotune.candidates_for(17)
-    assert {c["layout"] for c in big} == {"lane"}
+    assert {c["layout"] for c in autotune.candidates_for(24)} == {"lane", "group"}
+    assert {c["layout"] for c in autotune.candidates_for(25)} == {"lane"}
 
 
 def test_pairs_are_per_lane_candidates():

@@ -1,4 +1,4 @@
-"""The lane-group layout (one path per 16 lanes, hcg_build_options.layout = group) against the
+"""The lane-group layout (one path per 16 lanes, 32 for n > 16; layout = group) against the
 oracle on the GPU parity cases: identical return codes, singular flags, winding numbers,
 accepted and rejected step counts, and endpoints bit for bit (the layout performs the
 oracle's operations in the oracle's order), plus the mixed-size multi-system launch and the
@@ -141,11 +141,47 @@ def test_group_learned_equals_lane_learned(gpu_ctx):
     np.testing.assert_array_equal(grp.solution, lane.solution)
 
 
-def test_group_refuses_large_systems(gpu_ctx):
-    from hcgpu.abi import HcgpuError
-    F, p, q, _ = systems.four_bar()
-    with pytest.raises(HcgpuError, match="up to 16 variables"):
-        gpu_ctx.program(ParameterHomotopy(F, p, q), layout="group")
+def _four_bar_batch(m, seed=6):
+    F, p, q, s = systems.four_bar()
+    rng = np.random.default_rng(seed)
+    Q = np.asarray(q)[None, :] * (1 + 1e-3 * (rng.standard_normal((m, len(q))) + 1j * rng.standard_normal((m, len(q)))))
+    return ParameterHomotopy(F, p, Q), np.tile(np.asarray(s)[None, :], (m, 1)), np.arange(m, dtype=np.int32)
+
+
+def test_group32_four_bar_case(gpu_ctx):
+    """n = 24 on the 32-lane kernel: test/test_cases/four_bar.jl's 8-digit start ends in
+    success, equal to the oracle bit for bit."""
+    F, p, q, s = systems.four_bar()
+    H = ParameterHomotopy(F, p, q)
+    prog = gpu_ctx.program(H, layout="group")
+    assert prog.kernel_info()["workgroup"] == 256
+    print("group32: four-bar single path", flush=True)
+    g = track_batch(prog, np.asarray(s)[None, :], make_options())
+    assert g.codes() == ["success"]
+    assert_same(g, oracle.track_batch(H, np.asarray(s)[None, :], make_options()), "four-bar, group vs oracle")
+
+
+def test_group32_four_bar_batch_equals_oracle(gpu_ctx):
+    """256 nearby targets (the bench.py fourbar workload's first 256): every path equal to the
+    oracle's (codes, step counts, endpoints bit for bit).  With the learned proposer (which the
+    oracle does not run), the 32-lane kernel equals the per-lane kernel on the four shortest
+    paths."""
+    from hcgpu import learned
+    H, S, groups = _four_bar_batch(256)
+    prog = gpu_ctx.program(H, layout="group")
+    print("group32: four-bar batch, plain", flush=True)
+    g = track_batch(prog, S, make_options(), group_of_path=groups)
+    assert (g.code == 1).mean() > 0.9
+    o = oracle.track_batch(H, S, make_options(), group_of_path=groups)
+    assert_same(g, o, "four-bar batch, plain")
+    learned.load(gpu_ctx, learned.saved_weights("param10_step_mlp"))
+    idx = np.argsort(o.accepted_steps + o.rejected_steps, kind="stable")[:4]
+    print("group32: four-bar, learned, paths", idx.tolist(), flush=True)
+    gl = track_batch(prog, S[idx], make_options(learned_step=1), group_of_path=groups[idx])
+    ll = track_batch(gpu_ctx.program(H, layout="lane"), S[idx], make_options(learned_step=1), group_of_path=groups[idx])
+    assert gl.records["learned_proposed"].sum() > 0
+    assert_same(gl, ll, "four-bar, learned, group vs lane")
+    assert (gl.records["learned_accepted"] == ll.records["learned_accepted"]).all()
 
 
 def test_group_step_log_equals_oracle_log(gpu_ctx):

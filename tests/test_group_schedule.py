@@ -58,8 +58,12 @@ def test_every_op_type_and_quotients_schedule():
     assert info["nn"] == 16
 
 
-def test_systems_beyond_16_variables_are_refused():
-    from hcgpu.abi import HcgpuError
+def test_four_bar_takes_the_32_lane_kernel():
     F, p, q, _ = systems.four_bar()  # n = 24
-    with pytest.raises(HcgpuError, match="up to 16 variables"):
-        group_schedule_info([ParameterHomotopy(F, p, q)])
+    assert group_schedule_info([ParameterHomotopy(F, p, q)])["nn"] == 24
+
+
+def test_systems_beyond_24_variables_are_refused():
+    from hcgpu.abi import HcgpuError
+    with pytest.raises(HcgpuError, match="out of range"):
+        group_schedule_info([total_degree(dense_system(25, 1, seed=1), seed=2)[0]])
