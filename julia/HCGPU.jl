@@ -464,16 +464,21 @@ mutable struct GPUTracker <: HC.AbstractPathTracker
 end
 
 """
-    GPUTracker(T::EndgameTracker; device = 0, layout = :lane, learned_weights = nothing)
+    GPUTracker(T::EndgameTracker; device = 0, layout = :auto, learned_weights = nothing)
 
 Compile `T`'s homotopy for the GPU (`hcg_program_create_ex`): `layout = :lane` generates and
 compiles the tapes (one path per lane), `:group` schedules them for the precompiled lane-group
-kernel (one path per 16 lanes, 32 lanes for 17 to 24 variables).  `learned_weights`
+kernel (one path per 16 lanes, 32 lanes for 17 to 24 variables); `:auto` takes `:lane` up to
+12 variables, where the per-lane kernel factors the Jacobian in registers, and `:group` past
+that.  `learned_weights`
 (`HCG_MLP_NPARAMS` Float32) turns on the learned step proposer.
 """
-function GPUTracker(T::HC.EndgameTracker; device::Integer = 0, layout::Symbol = :lane,
+function GPUTracker(T::HC.EndgameTracker; device::Integer = 0, layout::Symbol = :auto,
                     learned_weights::Union{Nothing,Vector{Float32}} = nothing)
     hd = HomotopyData(T.tracker.homotopy)
+    if layout === :auto
+        layout = hd.desc[].n > 12 ? :group : :lane
+    end
     ctx = Ref{Ptr{Cvoid}}(C_NULL)
     check(hcg_ctx_create(device, ctx))
     prog = Ref{Ptr{Cvoid}}(C_NULL)

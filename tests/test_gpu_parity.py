@@ -20,9 +20,11 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-10
 
 
-def run_both(H, S, gpu_ctx, **opt):
+def run_both(H, S, gpu_ctx, layout=None, **opt):
     opts = make_options(**opt)
     T = EndgameTracker(H, ctx=gpu_ctx)
+    if layout is not None:
+        T._program = gpu_ctx.program(H, layout=layout)
     T.options = opts
     g = T.track_batch(S)
     o = oracle.track_batch(H, S, opts)
@@ -125,24 +127,26 @@ def _isapprox(a, b, rtol=np.sqrt(np.finfo(float).eps)):
     return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
 
 
-def test_steiner_parameter_case(gpu_ctx):
+@pytest.mark.parametrize("layout", ["lane", "group"])
+def test_steiner_parameter_case(gpu_ctx, layout):
     """test/test_cases/steiner_higher_prec.jl on the GPU (n = 15: the generic-size linear
     algebra): s_p at p reaches s_q at q, and s_q reaches s_p on the reversed homotopy, with
     extended-precision residuals on the way; results equal the oracle's."""
     F, p, q, s_p, s_q = systems.steiner()
     for H, start, want in ((ParameterHomotopy(F, p, q), s_p, s_q), (ParameterHomotopy(F, q, p), s_q, s_p)):
-        g, o = run_both(H, start[None, :], gpu_ctx)
+        g, o = run_both(H, start[None, :], gpu_ctx, layout=layout)
         assert_parity(g, o)
         assert g.codes() == ["success"]
         assert _isapprox(g.solution[0], want)
         assert g.extended_precision_used[0] == 1
 
 
-def test_four_bar_parameter_case(gpu_ctx):
+@pytest.mark.parametrize("layout", ["lane", "group"])
+def test_four_bar_parameter_case(gpu_ctx, layout):
     """test/test_cases/four_bar.jl on the GPU (n = 24, the largest supported size): the
     8-digit start point ends in success, as on the oracle."""
     F, p, q, s = systems.four_bar()
-    g, o = run_both(ParameterHomotopy(F, p, q), s[None, :], gpu_ctx)
+    g, o = run_both(ParameterHomotopy(F, p, q), s[None, :], gpu_ctx, layout=layout)
     assert_parity(g, o)
     assert g.codes() == ["success"]
 
