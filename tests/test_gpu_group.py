@@ -141,6 +141,17 @@ def test_group_learned_equals_lane_learned(gpu_ctx):
     np.testing.assert_array_equal(grp.solution, lane.solution)
 
 
+def test_default_layout_by_size(gpu_ctx):
+    """Without a tuned choice, GpuContext.program() takes one path per lane up to 12 variables
+    and the lane-group layout past that (where the per-lane LU leaves the registers)."""
+    F, p, q, _ = systems.four_bar()
+    assert gpu_ctx.program(ParameterHomotopy(F, p, q)).layout == "group"
+    F, p, q, _, _ = systems.steiner()
+    assert gpu_ctx.program(ParameterHomotopy(F, p, q)).layout == "group"
+    H, _ = total_degree(dense_system(4, 3, seed=100), seed=101)
+    assert gpu_ctx.program(H).layout == "lane"
+
+
 def _four_bar_batch(m, seed=6):
     F, p, q, s = systems.four_bar()
     rng = np.random.default_rng(seed)
