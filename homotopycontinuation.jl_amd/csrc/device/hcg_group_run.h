@@ -109,12 +109,12 @@ __device__ __forceinline__ void g_do_eval(GT<NN>& T) {
     T.req = GREQ_NONE;
 }
 
-// The request type a wave serves in this round of its path loop.  The four groups of a wave
+// The request type a wave serves in this pass of its path loop.  The four groups of a wave
 // run a slot program together only when their requests agree; groups with another pending
-// request wait for a later round.  Corrector requests come first and the predictor's Taylor
+// request wait for a later pass.  Corrector requests come first and the predictor's Taylor
 // requests last, so a group that has finished its step waits at the next step's first Taylor
 // request until the other groups of its wave arrive there, and from then on the groups step
-// together.  Every round serves at least one group (the type is one that some group is
+// together.  Every pass serves at least one group (the type is one that some group is
 // waiting with), and a waiting group waits at most until each other group of its wave reaches
 // its next Taylor request or ends its path.  Only the order of evaluations across groups
 // changes, never a path's own sequence, so results do not change.
@@ -246,7 +246,7 @@ __device__ __forceinline__ void hcg_group_run(const hcg_multi_kargs& m, const hc
             T.co_addsmp = T.co_sw = T.co_ainf = 0;
             evals = 0;
         }
-        // a request the wave did not serve in the last round is still pending
+        // a request the wave did not serve in the last pass is still pending
         const bool live = T.req != GREQ_NONE || T.path_run(m.sys[s].t1);
         if (live) {
             const int pick = g_wave_pick(T.req);
