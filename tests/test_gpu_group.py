@@ -195,6 +195,39 @@ def test_group32_four_bar_batch_equals_oracle(gpu_ctx):
     assert (gl.records["learned_accepted"] == ll.records["learned_accepted"]).all()
 
 
+def _chain20(seed=3):
+    """n = 20: 18 linear equations chaining the variables and two quadratics (4 total-degree
+    paths); a straight-line homotopy on the 32-lane kernel."""
+    from hcgpu.polys import System, variables
+    n = 20
+    xs, _ = variables(n)
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    eqs = [xs[i] - c[i] * xs[i + 1] - b[i] for i in range(n - 2)]
+    eqs += [xs[n - 2] ** 2 + c[n - 2] * xs[n - 1] - b[n - 2], xs[n - 1] ** 2 + c[n - 1] * xs[0] * xs[n - 2] - b[n - 1]]
+    return System(eqs, n)
+
+
+def test_group32_straight_line_case(gpu_ctx):
+    H, S = total_degree(_chain20(), seed=4)
+    print("group32: straight-line n = 20", flush=True)
+    g, o = run_group(gpu_ctx, H, S)
+    assert g.codes() == ["success"] * 4
+    assert_same(g, o, "chain n=20")
+
+
+def test_group32_multi_launch_with_a_small_system(gpu_ctx):
+    """A launch over n = 4 and n = 20 runs on the 32-lane kernel (the largest system decides),
+    where the n = 4 paths leave 28 lanes of their group idle; both equal the oracle."""
+    H4, S4 = total_degree(dense_system(4, 3, seed=100), seed=101)
+    H20, S20 = total_degree(_chain20(), seed=4)
+    print("group32: multi-system launch n = 4, 20", flush=True)
+    outs = track_multi(gpu_ctx.program_multi([H4, H20], layout="group"), [S4, S20], make_options())
+    assert_same(outs[0], oracle.track_batch(H4, S4, make_options()), "n=4 in the 32-lane kernel")
+    assert_same(outs[1], oracle.track_batch(H20, S20, make_options()), "n=20 beside n=4")
+
+
 def test_group_step_log_equals_oracle_log(gpu_ctx):
     """The step records (HCG_LOG_* layout) written by the group layout equal the oracle's record
     for record, on paths that reach the singular endgame (winding number 3)."""
